@@ -1,0 +1,1638 @@
+// d2bi_gpu.hip -- gfx950 kernels and the device half of the C-ABI.
+//
+// Stages of one imaging condition (one "field"):
+//   (a) binning      findGdotR + LinkCell3D add as a stable counting sort of
+//                    the 27 x N replica predicate into link cells
+//   (b) voxelise     per voxel S = sum_j exp(-r_j^2/2s^2 + i g.r_j) over the
+//                    27 neighbour cells, rho from the nearest range
+//   (c) integrate    RK4 Howie-Whelan march down each pixel column
+//   (d) blur         separable Gaussian on the output image
+// FP64 mode reproduces the reference arithmetic (un-fused, same order).
+// FAST mode runs (b) and (c) in FP32 with FP64 phases and an FP64 re-test
+// of every candidate whose FP32 range lies in a guard band around 3 sigma,
+// so voxel membership stays identical to FP64.
+//
+// This file is compiled with -ffp-contract=off: every fused multiply-add in
+// it is written explicitly (fmaf) and only appears in FAST-mode FP32 code.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/d2bi_gpu.h"
+#include "d2bi_internal.h"
+
+namespace d2bi {
+
+// ------------------------------------------------------------------ errors
+
+static char g_err[1024] = "";
+
+int seterr(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return seterr(D2BI_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                           \
+    } while (0)
+
+#define CHK(call)                        \
+    do {                                 \
+        int r_ = (call);                 \
+        if (r_ != D2BI_OK) return r_;    \
+    } while (0)
+
+// ------------------------------------------------------------------ device buffers
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return D2BI_OK;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t want = std::max<size_t>(bytes, 256);
+        want = want + want / 8;                      // headroom for slightly larger conditions
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return seterr(D2BI_ERR_HIP, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        }
+        cap = want;
+        return D2BI_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Workspace {
+    // binning
+    Buf x;                      // atoms (temporary upload)
+    Buf mask, blk_cnt, blk_off;
+    Buf pt_cell, pt_slot, pt_frac, pt_gdotr;
+    Buf cell_cnt, cell_start, cell_ids;
+    Buf slot_id, slot_frac, slot_cs64, slot_cs32;
+    Buf scan_tmp;
+    Buf counters;               // misc device counters (guard count, max per cell)
+    // voxel axis tables
+    Buf ax_cell, ax_zi, ax_start;
+    // fields and image
+    Buf f64_re, f64_im, f64_rho, f32_field;
+    Buf cnt, r2m;
+    Buf img;
+    Buf blur_in, blur_tmp, blur_out, blur_w, blur_k;
+    // pinned host staging
+    void* pinned = nullptr;
+    size_t pinned_cap = 0;
+    void release_all()
+    {
+        Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr,
+                      &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
+                      &slot_cs32, &scan_tmp, &counters, &ax_cell, &ax_zi, &ax_start, &f64_re,
+                      &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
+                      &blur_out, &blur_w, &blur_k};
+        for (Buf* b : all) b->release();
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_cap = 0;
+    }
+};
+
+struct State {
+    int device = -1;
+    bool init = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    Workspace ws;
+};
+
+static State g;
+
+static int need_init()
+{
+    if (!g.init) return seterr(D2BI_ERR_NOINIT, "d2bi_gpu_init has not been called");
+    return D2BI_OK;
+}
+
+static int pinned(size_t bytes, void** out)
+{
+    if (bytes > g.ws.pinned_cap) {
+        if (g.ws.pinned) (void)hipHostFree(g.ws.pinned);
+        g.ws.pinned = nullptr;
+        g.ws.pinned_cap = 0;
+        HIPCHK(hipHostMalloc(&g.ws.pinned, bytes, hipHostMallocDefault));
+        g.ws.pinned_cap = bytes;
+    }
+    *out = g.ws.pinned;
+    return D2BI_OK;
+}
+
+// ------------------------------------------------------------------ device atoms
+
+}  // namespace d2bi
+
+struct d2bi_atoms_s {
+    double* x = nullptr;        // 3*n doubles, AoS
+    int64_t n = 0;
+};
+
+namespace d2bi {
+
+// ------------------------------------------------------------------ generic scan
+
+static constexpr int SCAN_BLOCK = 256;
+static constexpr int SCAN_ITEMS = 4;
+static constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+// Block-wide exclusive scan of one int per thread (wave64 shuffles + LDS).
+__device__ inline int block_exclusive_scan(int v, int* total, int* lds /*[SCAN_BLOCK/64 + 1]*/)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) lds[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) { int t = lds[w]; lds[w] = s; s += t; }
+        lds[blockDim.x >> 6] = s;
+    }
+    __syncthreads();
+    int r = incl - v + lds[wave];
+    *total = lds[blockDim.x >> 6];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const int* __restrict__ in, int64_t n,
+                                                            int* __restrict__ sums)
+{
+    __shared__ int lds[SCAN_BLOCK / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++)
+        if (base + i < n) s += in[base + i];
+    int tot;
+    block_exclusive_scan(s, &tot, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Single-block exclusive scan of the block sums (any length), in place.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int* __restrict__ sums, int64_t nb,
+                                                          int* __restrict__ total_out)
+{
+    __shared__ int lds[SCAN_BLOCK / 64 + 1];
+    int carry = 0;
+    for (int64_t b = 0; b < nb; b += SCAN_BLOCK) {
+        int64_t i = b + threadIdx.x;
+        int v = i < nb ? sums[i] : 0;
+        int tot;
+        int e = block_exclusive_scan(v, &tot, lds);
+        if (i < nb) sums[i] = e + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const int* __restrict__ in, int64_t n,
+                                                           const int* __restrict__ sums,
+                                                           int* __restrict__ out)
+{
+    __shared__ int lds[SCAN_BLOCK / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        v[i] = base + i < n ? in[base + i] : 0;
+        s += v[i];
+    }
+    int tot;
+    int e = block_exclusive_scan(s, &tot, lds) + sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        if (base + i < n) out[base + i] = e;
+        e += v[i];
+    }
+}
+
+// out[0..n-1] = exclusive scan of in, out[n] = total.  in may equal out.
+static int scan_exclusive(const int* in, int* out, int64_t n, hipStream_t s)
+{
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb < 1) nb = 1;
+    CHK(g.ws.scan_tmp.ensure(sizeof(int) * (nb + 1)));
+    int* sums = g.ws.scan_tmp.as<int>();
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb, out + n);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out);
+    HIPCHK(hipGetLastError());
+    return D2BI_OK;
+}
+
+// ------------------------------------------------------------------ (a) binning
+
+static constexpr int BIN_BLOCK = 256;
+static constexpr int BIN_ROUNDS = 4;
+static constexpr int BIN_TILE = BIN_BLOCK * BIN_ROUNDS;
+static constexpr int MAX_REPL = 32;
+
+struct BinParams {
+    double RU[9];               // column-major
+    double delta[3];
+    double lc_delta[3];
+    double ia[9];               // link-cell inverse (column-major)
+    double gg[3];
+    double Ruvw[3 * MAX_REPL];
+    int nuvw;
+    int Ex, Ey, Ez;
+    int64_t n;                  // atoms
+    int generic;                // 1: LinkCell3D add1 (no transform, no filter)
+};
+
+// Link-cell index of a reduced coordinate: mod(floor(y) + 16*N, N)
+// (whichCell, Lib_SimpleSupercells.f90:371-373).  For points further than
+// 16 boxes away the reference's expression goes negative; here the floor is
+// clamped and the modulo kept non-negative, which changes nothing for
+// points within range and keeps every index inside the cell array.
+__device__ inline int cell_of(double fl, int n)
+{
+    const double c = fmin(fmax(fl, -1.0e9), 1.0e9);
+    const int r = ((int)c + 16 * n) % n;
+    return r < 0 ? r + n : r;
+}
+
+// Transform atom j for replica nn: yy (rotated lab position), zz (link-cell
+// frame), and whether it lies in the link-cell box.  Operation order of
+// findGdotR (Lib_DynamicalTwoBeamImaging.f90:1225-1240) and pointInSupercell0
+// (Lib_SimpleSupercells.f90:1077-1084).
+__device__ inline bool bin_point(const BinParams& P, const double* x3, int nn, double yy[3],
+                                 double cyy[3])
+{
+    if (P.generic) {
+        yy[0] = x3[0]; yy[1] = x3[1]; yy[2] = x3[2];
+        for (int i = 0; i < 3; i++)
+            cyy[i] = (P.ia[i] * yy[0] + P.ia[3 + i] * yy[1]) + P.ia[6 + i] * yy[2];
+        return true;
+    }
+    double y0[3], zz[3];
+    for (int i = 0; i < 3; i++) y0[i] = x3[i] + P.Ruvw[3 * nn + i];
+    for (int i = 0; i < 3; i++) yy[i] = (P.RU[i] * y0[0] + P.RU[3 + i] * y0[1]) + P.RU[6 + i] * y0[2];
+    for (int i = 0; i < 3; i++) {
+        yy[i] = yy[i] + P.delta[i];
+        zz[i] = yy[i] - P.lc_delta[i];
+    }
+    for (int i = 0; i < 3; i++)
+        cyy[i] = (P.ia[i] * zz[0] + P.ia[3 + i] * zz[1]) + P.ia[6 + i] * zz[2];
+    // |floor| beyond 2^30 cannot be inside the box; clamping keeps int conversion defined
+    int ix = (int)fmin(fmax(floor(cyy[0]), -1.0e9), 1.0e9);
+    int iy = (int)fmin(fmax(floor(cyy[1]), -1.0e9), 1.0e9);
+    int iz = (int)fmin(fmax(floor(cyy[2]), -1.0e9), 1.0e9);
+    bool is = (ix == (ix + 256 * P.Ex) % P.Ex);
+    is = is && (iy == (iy + 256 * P.Ey) % P.Ey);
+    is = is && (iz == (iz + 256 * P.Ez) % P.Ez);
+    return is;
+}
+
+// Pass 1: replica predicate bits per atom and per-(replica, block) counts.
+__global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __restrict__ x, BinParams P,
+                                                            uint32_t* __restrict__ mask,
+                                                            int* __restrict__ blk_cnt, int nblk)
+{
+    __shared__ int cnt[MAX_REPL];
+    if (threadIdx.x < MAX_REPL) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int r = 0; r < BIN_ROUNDS; r++) {
+        int64_t j = (int64_t)blockIdx.x * BIN_TILE + r * BIN_BLOCK + threadIdx.x;
+        uint32_t m = 0;
+        if (j < P.n) {
+            double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+            for (int nn = 0; nn < P.nuvw; nn++) {
+                double yy[3], cyy[3];
+                if (bin_point(P, x3, nn, yy, cyy)) m |= 1u << nn;
+            }
+            mask[j] = m;
+        }
+        for (int nn = 0; nn < P.nuvw; nn++) {
+            unsigned long long b = __ballot((m >> nn) & 1u);
+            if (lane == 0 && b) atomicAdd(&cnt[nn], (int)__popcll(b));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < P.nuvw) blk_cnt[(int64_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Pass 2: point number ii in (replica, atom) order, link cell, offset in
+// cell and g.r; per-cell arrival slot from an atomic counter.
+__global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restrict__ x, BinParams P,
+                                                           const uint32_t* __restrict__ mask,
+                                                           const int* __restrict__ blk_off, int nblk,
+                                                           int* __restrict__ pt_cell,
+                                                           int* __restrict__ pt_slot,
+                                                           double* __restrict__ pt_frac,
+                                                           double* __restrict__ pt_gdotr,
+                                                           int* __restrict__ cell_cnt)
+{
+    __shared__ int run[MAX_REPL];
+    __shared__ int wcnt[BIN_BLOCK / 64][MAX_REPL];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < P.nuvw) run[threadIdx.x] = blk_off[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < BIN_ROUNDS; r++) {
+        int64_t j = (int64_t)blockIdx.x * BIN_TILE + r * BIN_BLOCK + threadIdx.x;
+        uint32_t m = (j < P.n) ? mask[j] : 0u;
+        for (int nn = 0; nn < P.nuvw; nn++) {
+            unsigned long long b = __ballot((m >> nn) & 1u);
+            if (lane == 0) wcnt[wave][nn] = (int)__popcll(b);
+        }
+        __syncthreads();
+        if (m) {
+            double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+            for (int nn = 0; nn < P.nuvw; nn++) {
+                unsigned long long b = __ballot((m >> nn) & 1u);   // uniform among lanes with m != 0
+                if (!((m >> nn) & 1u)) continue;
+                int off = run[nn];
+                for (int w = 0; w < wave; w++) off += wcnt[w][nn];
+                int ii = off + (int)__popcll(b & lt);
+                double yy[3], cyy[3];
+                bin_point(P, x3, nn, yy, cyy);
+                double fl[3] = {floor(cyy[0]), floor(cyy[1]), floor(cyy[2])};
+                int cx = cell_of(fl[0], P.Ex);
+                int cy = cell_of(fl[1], P.Ey);
+                int cz = cell_of(fl[2], P.Ez);
+                int cell = cx + P.Ex * (cy + P.Ey * cz);
+                pt_cell[ii] = cell;
+                pt_frac[3 * (int64_t)ii + 0] = cyy[0] - fl[0];
+                pt_frac[3 * (int64_t)ii + 1] = cyy[1] - fl[1];
+                pt_frac[3 * (int64_t)ii + 2] = cyy[2] - fl[2];
+                pt_gdotr[ii] = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
+                pt_slot[ii] = atomicAdd(&cell_cnt[cell], 1);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < P.nuvw) {
+            int s = 0;
+            for (int w = 0; w < BIN_BLOCK / 64; w++) s += wcnt[w][threadIdx.x];
+            run[threadIdx.x] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// Pass 3: place each point at its cell's start + arrival slot.
+__global__ void k_bin_place(int64_t nr, const int* __restrict__ pt_cell, const int* __restrict__ pt_slot,
+                            const int* __restrict__ cell_start, int* __restrict__ cell_ids)
+{
+    int64_t ii = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= nr) return;
+    cell_ids[cell_start[pt_cell[ii]] + pt_slot[ii]] = (int)ii;
+}
+
+// Pass 4: restore insertion order inside each cell (sort by ii, which is
+// the reference's add order) and gather slot-ordered point data: 1-based
+// id, FP64 offset in cell, FP64 and FP32 (cos, sin) of g.r.
+__global__ void k_bin_finish(int ncell, const int* __restrict__ cell_start, int* __restrict__ cell_ids,
+                             const double* __restrict__ pt_frac, const double* __restrict__ pt_gdotr,
+                             int* __restrict__ slot_id, double4* __restrict__ slot_frac,
+                             double2* __restrict__ slot_cs64, float2* __restrict__ slot_cs32,
+                             int* __restrict__ max_per_cell)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int s0 = cell_start[c], s1 = cell_start[c + 1];
+    for (int i = s0 + 1; i < s1; i++) {                 // insertion sort (cells hold a few points)
+        int v = cell_ids[i];
+        int k = i - 1;
+        while (k >= s0 && cell_ids[k] > v) { cell_ids[k + 1] = cell_ids[k]; k--; }
+        cell_ids[k + 1] = v;
+    }
+    for (int s = s0; s < s1; s++) {
+        int ii = cell_ids[s];
+        slot_id[s] = ii + 1;
+        slot_frac[s] = make_double4(pt_frac[3 * (int64_t)ii], pt_frac[3 * (int64_t)ii + 1],
+                                    pt_frac[3 * (int64_t)ii + 2], 0.0);
+        double sn, cs;
+        sincos(pt_gdotr[ii], &sn, &cs);
+        slot_cs64[s] = make_double2(cs, sn);
+        slot_cs32[s] = make_float2((float)cs, (float)sn);
+    }
+    if (s1 - s0 > 0) atomicMax(max_per_cell, s1 - s0);
+}
+
+struct Binned {
+    int64_t nr = 0;
+    int ncell = 0;
+    int max_per_cell = 0;
+};
+
+// Run binning for either a condition (generic = 0) or plain points.
+static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, hipStream_t s)
+{
+    Workspace& w = g.ws;
+    const int64_t ncell64 = (int64_t)P.Ex * P.Ey * P.Ez;
+    if (ncell64 > (int64_t)INT32_MAX / 2) return seterr(D2BI_ERR_CAPACITY, "too many link cells");
+    const int ncell = (int)ncell64;
+    const int nblk = (int)std::max<int64_t>(1, (n + BIN_TILE - 1) / BIN_TILE);
+    P.n = n;
+    CHK(w.mask.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1)));
+    CHK(w.blk_cnt.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
+    CHK(w.blk_off.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
+    CHK(w.cell_cnt.ensure(sizeof(int) * (ncell + 1)));
+    CHK(w.cell_start.ensure(sizeof(int) * (ncell + 1)));
+    CHK(w.counters.ensure(64));
+    HIPCHK(hipMemsetAsync(w.cell_cnt.p, 0, sizeof(int) * (ncell + 1), s));
+    HIPCHK(hipMemsetAsync(w.counters.p, 0, 64, s));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_bin_classify, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
+                           w.mask.as<uint32_t>(), w.blk_cnt.as<int>(), nblk);
+        HIPCHK(hipGetLastError());
+    } else {
+        HIPCHK(hipMemsetAsync(w.blk_cnt.p, 0, sizeof(int) * ((int64_t)P.nuvw * nblk + 1), s));
+    }
+    const int64_t nbc = (int64_t)P.nuvw * nblk;
+    CHK(scan_exclusive(w.blk_cnt.as<int>(), w.blk_off.as<int>(), nbc, s));
+    int nr_host = 0;
+    HIPCHK(hipMemcpyAsync(&nr_host, w.blk_off.as<int>() + nbc, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (nr_host < 0) return seterr(D2BI_ERR_CAPACITY, "point count overflow");
+    const int64_t nr = nr_host;
+    const int64_t nrc = std::max<int64_t>(nr, 1);
+    CHK(w.pt_cell.ensure(sizeof(int) * nrc));
+    CHK(w.pt_slot.ensure(sizeof(int) * nrc));
+    CHK(w.pt_frac.ensure(sizeof(double) * 3 * nrc));
+    CHK(w.pt_gdotr.ensure(sizeof(double) * nrc));
+    CHK(w.cell_ids.ensure(sizeof(int) * nrc));
+    CHK(w.slot_id.ensure(sizeof(int) * nrc));
+    CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
+    CHK(w.slot_cs64.ensure(sizeof(double2) * nrc));
+    CHK(w.slot_cs32.ensure(sizeof(float2) * nrc));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
+                           w.mask.as<uint32_t>(), w.blk_off.as<int>(), nblk, w.pt_cell.as<int>(),
+                           w.pt_slot.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(),
+                           w.cell_cnt.as<int>());
+        HIPCHK(hipGetLastError());
+    }
+    CHK(scan_exclusive(w.cell_cnt.as<int>(), w.cell_start.as<int>(), ncell, s));
+    if (nr > 0) {
+        hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
+                           w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.cell_start.as<int>(),
+                           w.cell_ids.as<int>());
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_bin_finish, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell,
+                       w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
+                       w.pt_gdotr.as<double>(), w.slot_id.as<int>(), w.slot_frac.as<double4>(),
+                       w.slot_cs64.as<double2>(), w.slot_cs32.as<float2>(), w.counters.as<int>());
+    HIPCHK(hipGetLastError());
+    out->nr = nr;
+    out->ncell = ncell;
+    return D2BI_OK;
+}
+
+// Fill the binning parameters of a condition from the plan.
+static int bin_params(const d2bi_record* r, const d2bi_plan* p, BinParams* P)
+{
+    std::memset(P, 0, sizeof *P);
+    if (r->nuvw < 1 || r->nuvw > MAX_REPL)
+        return seterr(D2BI_ERR_ARG, "nuvw = %d not supported (1..%d)", r->nuvw, MAX_REPL);
+    std::memcpy(P->RU, p->RU, sizeof P->RU);
+    std::memcpy(P->delta, r->delta, sizeof P->delta);
+    std::memcpy(P->lc_delta, p->lc_delta, sizeof P->lc_delta);
+    std::memcpy(P->ia, p->ia_col, sizeof P->ia);
+    std::memcpy(P->gg, p->gg, sizeof P->gg);
+    for (int nn = 0; nn < r->nuvw; nn++) {
+        const int* u = &r->uvw[3 * nn];
+        for (int i = 0; i < 3; i++) {       // getSuperVector (Lib_SimpleSupercells.f90:904-918)
+            double s1 = r->super_a[i] * r->super_n[0];
+            double s2 = r->super_a[3 + i] * r->super_n[1];
+            double s3 = r->super_a[6 + i] * r->super_n[2];
+            P->Ruvw[3 * nn + i] = (u[0] * s1 + u[1] * s2) + u[2] * s3;
+        }
+    }
+    P->nuvw = r->nuvw;
+    P->Ex = p->Ex; P->Ey = p->Ey; P->Ez = p->Ez;
+    P->generic = 0;
+    return D2BI_OK;
+}
+
+// ------------------------------------------------------------------ voxel axis tables
+
+// The voxel and link cells are cuboidal (findTransformedSupercell gives a
+// diagonal D, Lib_DiffractionConditions.f90:1074-1077), so a voxel's link
+// cell and in-cell offset factorise per axis: for axis x,
+//   xx = a_vox(1,1)*ix - lc_delta(1),  zi = ia(1,1)*xx,
+//   cell = mod(floor(zi) + 16*Ex, Ex), zi = zi - cell
+// (neighbourList1b, Lib_LinkCell3D.f90:1549-1565).  The zero off-diagonal
+// terms of the reference's 3x3 products add +-0 and change no bit.
+struct AxisTables {
+    std::vector<int> cell[3];
+    std::vector<double> zi[3];
+    std::vector<int> start[3];     // first voxel of each link cell (size E+1)
+    int lo[3], hi[3];              // active link-cell range
+};
+
+static bool diagonal(const double* m)
+{
+    return m[1] == 0.0 && m[2] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[6] == 0.0 && m[7] == 0.0;
+}
+
+static int axis_tables(const d2bi_record* r, const d2bi_plan* p, AxisTables* t)
+{
+    if (!diagonal(r->gk_a) || !diagonal(p->a_col) || !diagonal(p->ia_col))
+        return seterr(D2BI_ERR_ARG, "voxel/link cells must be cuboidal (diagonal cell matrices)");
+    const int M[3] = {p->Mx, p->My, p->Mz};
+    const int E[3] = {p->Ex, p->Ey, p->Ez};
+    for (int a = 0; a < 3; a++) {
+        t->cell[a].resize(M[a]);
+        t->zi[a].resize(M[a]);
+        t->start[a].assign(E[a] + 1, M[a]);
+        const double av = r->gk_a[4 * a], ia = p->ia_col[4 * a];
+        for (int i = 0; i < M[a]; i++) {
+            double xx = av * i - p->lc_delta[a];
+            double zi = ia * xx;
+            int c = ((int)floor(zi) + 16 * E[a]) % E[a];
+            t->cell[a][i] = c;
+            t->zi[a][i] = zi - c;
+        }
+        for (int i = 1; i < M[a]; i++)
+            if (t->cell[a][i] < t->cell[a][i - 1])
+                return seterr(D2BI_ERR_ARG, "voxel column wraps the link-cell box (axis %d)", a);
+        for (int i = M[a] - 1; i >= 0; i--) t->start[a][t->cell[a][i]] = i;
+        // cells with no voxels take the start of the next occupied cell
+        for (int c = E[a] - 1; c >= 0; c--)
+            if (t->start[a][c] > t->start[a][c + 1]) t->start[a][c] = t->start[a][c + 1];
+        t->lo[a] = t->cell[a][0];
+        t->hi[a] = t->cell[a][M[a] - 1];
+    }
+    return D2BI_OK;
+}
+
+// Upload tables: ints [cellx|celly|cellz], doubles [zix|ziy|ziz], ints [startx|starty|startz]
+struct AxisDev {
+    const int* cell[3];
+    const double* zi[3];
+    const int* start[3];
+};
+
+static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
+{
+    Workspace& w = g.ws;
+    size_t nc = t.cell[0].size() + t.cell[1].size() + t.cell[2].size();
+    size_t ns = t.start[0].size() + t.start[1].size() + t.start[2].size();
+    CHK(w.ax_cell.ensure(sizeof(int) * nc));
+    CHK(w.ax_zi.ensure(sizeof(double) * nc));
+    CHK(w.ax_start.ensure(sizeof(int) * ns));
+    size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns;
+    void* hp;
+    CHK(pinned(bytes, &hp));
+    char* h = (char*)hp;
+    int* hc = (int*)h;
+    double* hz = (double*)(h + sizeof(int) * nc);
+    int* hs = (int*)(h + sizeof(int) * nc + sizeof(double) * nc);
+    size_t oc = 0, os = 0;
+    for (int a = 0; a < 3; a++) {
+        std::memcpy(hc + oc, t.cell[a].data(), sizeof(int) * t.cell[a].size());
+        std::memcpy(hz + oc, t.zi[a].data(), sizeof(double) * t.zi[a].size());
+        std::memcpy(hs + os, t.start[a].data(), sizeof(int) * t.start[a].size());
+        d->cell[a] = w.ax_cell.as<int>() + oc;
+        d->zi[a] = w.ax_zi.as<double>() + oc;
+        d->start[a] = w.ax_start.as<int>() + os;
+        oc += t.cell[a].size();
+        os += t.start[a].size();
+    }
+    HIPCHK(hipMemcpyAsync(w.ax_cell.p, hc, sizeof(int) * nc, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.ax_zi.p, hz, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.ax_start.p, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
+    // the pinned buffer is reused by later calls: wait until the copies are done
+    HIPCHK(hipStreamSynchronize(s));
+    return D2BI_OK;
+}
+
+struct VoxParams {
+    int Mx, My, Mz;
+    int Ex, Ey, Ez;
+    double a[3];                // link-cell sides (diagonal of a_col)
+    double rc2, i2s2, sigma;
+    int use_density;
+    AxisDev ax;
+};
+
+// convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994)
+__device__ inline double rho_of(double r2_min, double sig)
+{
+    if (r2_min < sig * sig) return 1.0;
+    if (r2_min > 4 * sig * sig) return 0.0;
+    double xx = sqrt(r2_min) / sig - 1.0;
+    return 1 - ((xx * xx) * xx) * ((10 - 15 * xx) + (6 * xx) * xx);
+}
+
+// ------------------------------------------------------------------ (b) voxelise, FP64
+
+// One thread per voxel; the 27 neighbour cells in (kz,ky,kx) order and the
+// points of each cell in insertion order, exactly the neighbour list order
+// of neighbourList1b, summed as the reference sums it (:1877-1892).
+__global__ __launch_bounds__(256) void k_voxel_fp64(VoxParams P, const int* __restrict__ cell_start,
+                                                    const double4* __restrict__ frac,
+                                                    const double2* __restrict__ cs64,
+                                                    double* __restrict__ out_re,
+                                                    double* __restrict__ out_im,
+                                                    double* __restrict__ out_rho,
+                                                    int* __restrict__ out_cnt,
+                                                    double* __restrict__ out_r2m)
+{
+    const int ix = blockIdx.x * 8 + (threadIdx.x & 7);
+    const int iy = blockIdx.y * 8 + ((threadIdx.x >> 3) & 7);
+    const int iz = blockIdx.z * 4 + (threadIdx.x >> 6);
+    if (ix >= P.Mx || iy >= P.My || iz >= P.Mz) return;
+    const int cx = P.ax.cell[0][ix], cy = P.ax.cell[1][iy], cz = P.ax.cell[2][iz];
+    const double zx = P.ax.zi[0][ix], zy = P.ax.zi[1][iy], zz = P.ax.zi[2][iz];
+    double sre = 0.0, sim = 0.0, r2min = 3.4028234663852886e38;
+    int n = 0;
+    for (int kz = -1; kz <= 1; kz++) {
+        const int jz = (cz + P.Ez + kz) % P.Ez;
+        const double dzi3 = kz - zz;
+        for (int ky = -1; ky <= 1; ky++) {
+            const int jy = (cy + P.Ey + ky) % P.Ey;
+            const double dzi2 = ky - zy;
+            for (int kx = -1; kx <= 1; kx++) {
+                const int jx = (cx + P.Ex + kx) % P.Ex;
+                const double dzi1 = kx - zx;
+                const int c = jx + P.Ex * (jy + P.Ey * jz);
+                const int s1 = cell_start[c + 1];
+                for (int s = cell_start[c]; s < s1; s++) {
+                    const double4 f = frac[s];
+                    const double d1 = P.a[0] * (f.x + dzi1);
+                    const double d2 = P.a[1] * (f.y + dzi2);
+                    const double d3 = P.a[2] * (f.z + dzi3);
+                    const double dd = (d1 * d1 + d2 * d2) + d3 * d3;
+                    if (dd <= P.rc2) {
+                        n++;
+                        r2min = r2min < dd ? r2min : dd;
+                        const double e = exp(-dd * P.i2s2);
+                        const double2 cs = cs64[s];
+                        sre = sre + e * cs.x;
+                        sim = sim + e * cs.y;
+                    }
+                }
+            }
+        }
+    }
+    const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
+    double ww = hypot(sre, sim);
+    ww = 1 / (1.0e-16 > ww ? 1.0e-16 : ww);
+    out_re[o] = sre * ww;
+    out_im[o] = sim * ww;
+    out_rho[o] = P.use_density ? rho_of(r2min, P.sigma) : 1.0;
+    if (out_cnt) out_cnt[o] = n;
+    if (out_r2m) out_r2m[o] = r2min;
+}
+
+// ------------------------------------------------------------------ (c) integrate, FP64
+
+struct IntParams {
+    int Mx, My, Mz, nPrec;
+    double b0, bg, dz;
+    const double* aa;           // device array of 2 pi s_g per precession point
+};
+
+// dynamicalTwoBeamIntegrationColumn_phaserho (Lib_DynamicalTwoBeamImaging.f90:2316-2480):
+// one thread per pixel column, per-double-step renormalisation, precession
+// average (:1897-1900, 1951).
+__global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const double* __restrict__ fre,
+                                                        const double* __restrict__ fim,
+                                                        const double* __restrict__ frho,
+                                                        double* __restrict__ img)
+{
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)P.Mx * P.My;
+    if (col >= plane) return;
+    const int nz = P.Mz;
+    const double b0 = P.b0, bg = P.bg, dz = P.dz;
+    double acc = 0.0;
+    for (int prec = 0; prec < P.nPrec; prec++) {
+        const double a = P.aa[prec];
+        double phi0r = 1.0, phi0i = 0.0, phigr = 0.0, phigi = 0.0;
+        double dphi0r, dphi0i, dphigr, dphigi;
+        double phi0r1, phi0i1, phigr1, phigi1, phi0r2, phi0i2, phigr2, phigi2;
+        // coefficients at sample k (sample Mz is sample 0, :1894-1895)
+        auto coef = [&](int k, double& c0, double& ca, double& cgr, double& cgi) {
+            const int64_t o = col + plane * (k == nz ? 0 : k);
+            const double rh = frho[o];
+            c0 = b0 * rh;
+            ca = b0 * rh + a;
+            cgr = (bg * rh) * fre[o];
+            cgi = (bg * rh) * fim[o];
+        };
+        double c0a, caa, cgra, cgia;
+        if (nz >= 2) coef(0, c0a, caa, cgra, cgia);
+        for (int ii = 0; ii <= nz - 2; ii += 2) {
+            double c0b, cab, cgrb, cgib, c0c, cac, cgrc, cgic;
+            coef(ii + 1, c0b, cab, cgrb, cgib);
+            coef(ii + 2, c0c, cac, cgrc, cgic);
+#define DERIV(c0, ca, cgr, cgi)                                      \
+    dphi0r = ((-(c0 * phi0i)) - cgi * phigr) - cgr * phigi;          \
+    dphi0i = ((c0 * phi0r) + cgr * phigr) - cgi * phigi;             \
+    dphigr = ((cgi * phi0r) - cgr * phi0i) - ca * phigi;             \
+    dphigi = ((cgr * phi0r) + cgi * phi0i) + ca * phigr;
+            DERIV(c0a, caa, cgra, cgia)
+            phi0r2 = phi0r; phi0i2 = phi0i; phigr2 = phigr; phigi2 = phigi;
+            phi0r1 = phi0r + (dphi0r * dz) / 3; phi0i1 = phi0i + (dphi0i * dz) / 3;
+            phigr1 = phigr + (dphigr * dz) / 3; phigi1 = phigi + (dphigi * dz) / 3;
+            phi0r = phi0r + dphi0r * dz; phi0i = phi0i + dphi0i * dz;
+            phigr = phigr + dphigr * dz; phigi = phigi + dphigi * dz;
+            DERIV(c0b, cab, cgrb, cgib)
+            phi0r = phi0r2 + dphi0r * dz; phi0i = phi0i2 + dphi0i * dz;
+            phigr = phigr2 + dphigr * dz; phigi = phigi2 + dphigi * dz;
+            phi0r1 = phi0r1 + ((dphi0r * 2) * dz) / 3; phi0i1 = phi0i1 + ((dphi0i * 2) * dz) / 3;
+            phigr1 = phigr1 + ((dphigr * 2) * dz) / 3; phigi1 = phigi1 + ((dphigi * 2) * dz) / 3;
+            DERIV(c0b, cab, cgrb, cgib)
+            phi0r = phi0r2 + (dphi0r * 2) * dz; phi0i = phi0i2 + (dphi0i * 2) * dz;
+            phigr = phigr2 + (dphigr * 2) * dz; phigi = phigi2 + (dphigi * 2) * dz;
+            phi0r1 = phi0r1 + ((dphi0r * 2) * dz) / 3; phi0i1 = phi0i1 + ((dphi0i * 2) * dz) / 3;
+            phigr1 = phigr1 + ((dphigr * 2) * dz) / 3; phigi1 = phigi1 + ((dphigi * 2) * dz) / 3;
+            DERIV(c0c, cac, cgrc, cgic)
+#undef DERIV
+            phi0r = phi0r1 + (dphi0r * dz) / 3; phi0i = phi0i1 + (dphi0i * dz) / 3;
+            phigr = phigr1 + (dphigr * dz) / 3; phigi = phigi1 + (dphigi * dz) / 3;
+            double phimag = ((phi0r * phi0r + phi0i * phi0i) + phigr * phigr) + phigi * phigi;
+            phimag = 1 / sqrt(phimag);
+            phi0r = phi0r * phimag; phi0i = phi0i * phimag;
+            phigr = phigr * phimag; phigi = phigi * phimag;
+            c0a = c0c; caa = cac; cgra = cgrc; cgia = cgic;
+        }
+        acc = acc + (phigr * phigr + phigi * phigi);
+    }
+    img[col] = acc / (1 * P.nPrec);
+}
+
+// ------------------------------------------------------------------ (b) voxelise, FAST
+
+static constexpr int FAST_BLOCK = 256;
+static constexpr int FAST_VPT = 2;                    // voxels per thread per pass
+static constexpr int FAST_CHUNK = 512;                // staged candidates per pass
+
+struct FastParams {
+    VoxParams v;
+    int nax, nay, naz;          // active link cells per axis
+    int nblocks;                // nax*nay*naz
+    int per_xcd;                // blocks per XCD slice (XCD-aware swizzle)
+    float rc2_lo, rc2_hi;       // guard band around (3 sigma)^2
+    float k2;                   // -log2(e)/(2 sigma^2)
+};
+
+// One workgroup per link cell: every voxel of the cell shares the 27-cell
+// neighbourhood and its wrap offsets.  Candidates (positions relative to the
+// cell origin, FP64 -> FP32 after adding the neighbour offset, and FP32
+// cos/sin of g.r computed in FP64) are staged in LDS and read by all lanes
+// as broadcasts.  Candidates whose FP32 squared range falls within the
+// guard band of (3 sigma)^2 are re-tested with the FP64 expression of
+// neighbourList1b, so membership is exact.  S is Kahan-compensated.
+__global__ __launch_bounds__(FAST_BLOCK) void k_voxel_fast(FastParams F, const int* __restrict__ cell_start,
+                                                           const double4* __restrict__ frac,
+                                                           const float2* __restrict__ cs32,
+                                                           float4* __restrict__ field,
+                                                           int* __restrict__ out_cnt,
+                                                           float* __restrict__ out_r2m,
+                                                           unsigned long long* __restrict__ guard_count)
+{
+    // XCD-aware order: hardware dispatches consecutive workgroups round-robin
+    // over the 8 XCDs; give each XCD a contiguous slab of link cells so that
+    // neighbouring cells (which share 18 of 27 neighbour cells) hit one L2.
+    const int bid = blockIdx.x;
+    const int L = (bid & 7) * F.per_xcd + (bid >> 3);
+    if (L >= F.nblocks) return;
+    const VoxParams& P = F.v;
+    const int bx = L % F.nax, by = (L / F.nax) % F.nay, bz = L / (F.nax * F.nay);
+    const int cx = P.ax.cell[0][0] + bx, cy = P.ax.cell[1][0] + by, cz = P.ax.cell[2][0] + bz;
+    const int vx0 = P.ax.start[0][cx], vx1 = P.ax.start[0][cx + 1];
+    const int vy0 = P.ax.start[1][cy], vy1 = P.ax.start[1][cy + 1];
+    const int vz0 = P.ax.start[2][cz], vz1 = P.ax.start[2][cz + 1];
+    const int nvx = vx1 - vx0, nvy = vy1 - vy0, nvz = vz1 - vz0;
+    const int nvox = nvx * nvy * nvz;
+    if (nvox <= 0) return;
+
+    __shared__ int nb_start[27], nb_off[28];
+    __shared__ float4 sp[FAST_CHUNK];          // x, y, z (A, rel. cell origin), cos
+    __shared__ float ssn[FAST_CHUNK];          // sin
+    __shared__ int sslot[FAST_CHUNK];          // slot of the candidate
+    __shared__ unsigned char snb[FAST_CHUNK];  // its neighbour-cell index 0..26
+    const int tid = threadIdx.x;
+    if (tid < 27) {
+        const int kx = tid % 3 - 1, ky = (tid / 3) % 3 - 1, kz = tid / 9 - 1;
+        const int jx = (cx + P.Ex + kx) % P.Ex, jy = (cy + P.Ey + ky) % P.Ey, jz = (cz + P.Ez + kz) % P.Ez;
+        const int c = jx + P.Ex * (jy + P.Ey * jz);
+        nb_start[tid] = cell_start[c];
+        nb_off[tid + 1] = cell_start[c + 1] - cell_start[c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        nb_off[0] = 0;
+        for (int i = 1; i <= 27; i++) nb_off[i] += nb_off[i - 1];
+    }
+    __syncthreads();
+    const int T = nb_off[27];
+    unsigned long long nguard = 0;
+
+    for (int vbase = 0; vbase < nvox; vbase += FAST_BLOCK * FAST_VPT) {
+        float qx[FAST_VPT], qy[FAST_VPT], qz[FAST_VPT];
+        float sre[FAST_VPT], sim[FAST_VPT], cre[FAST_VPT], cim[FAST_VPT], r2m[FAST_VPT];
+        int cnt[FAST_VPT], lx[FAST_VPT], ly[FAST_VPT], lz[FAST_VPT];
+        bool act[FAST_VPT];
+#pragma unroll
+        for (int j = 0; j < FAST_VPT; j++) {
+            const int v = vbase + tid + j * FAST_BLOCK;
+            act[j] = v < nvox;
+            const int vv = act[j] ? v : 0;
+            lx[j] = vv % nvx; ly[j] = (vv / nvx) % nvy; lz[j] = vv / (nvx * nvy);
+            qx[j] = (float)(P.a[0] * P.ax.zi[0][vx0 + lx[j]]);
+            qy[j] = (float)(P.a[1] * P.ax.zi[1][vy0 + ly[j]]);
+            qz[j] = (float)(P.a[2] * P.ax.zi[2][vz0 + lz[j]]);
+            sre[j] = sim[j] = cre[j] = cim[j] = 0.0f;
+            r2m[j] = 3.4028234663852886e38f;
+            cnt[j] = 0;
+        }
+        for (int base = 0; base < T; base += FAST_CHUNK) {
+            const int nch = min(FAST_CHUNK, T - base);
+            __syncthreads();
+            for (int t = tid; t < nch; t += FAST_BLOCK) {
+                const int gi = base + t;
+                int nb = 0;
+                while (nb_off[nb + 1] <= gi) nb++;
+                const int slot = nb_start[nb] + (gi - nb_off[nb]);
+                const int kx = nb % 3 - 1, ky = (nb / 3) % 3 - 1, kz = nb / 9 - 1;
+                const double4 f = frac[slot];
+                const float2 c = cs32[slot];
+                sp[t] = make_float4((float)(P.a[0] * (f.x + kx)), (float)(P.a[1] * (f.y + ky)),
+                                    (float)(P.a[2] * (f.z + kz)), c.x);
+                ssn[t] = c.y;
+                sslot[t] = slot;
+                snb[t] = (unsigned char)nb;
+            }
+            __syncthreads();
+            for (int t = 0; t < nch; t++) {
+                const float4 pc = sp[t];
+#pragma unroll
+                for (int j = 0; j < FAST_VPT; j++) {
+                    const float dx = pc.x - qx[j], dy = pc.y - qy[j], dz = pc.z - qz[j];
+                    const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (act[j] && dd <= F.rc2_hi) {
+                        bool in = true;
+                        if (dd >= F.rc2_lo) {
+                            // exact FP64 test, neighbourList1b (Lib_LinkCell3D.f90:1579-1596)
+                            nguard++;
+                            const int sl = sslot[t], nb = snb[t];
+                            const double4 f = frac[sl];
+                            const double d1 = P.a[0] * (f.x + ((nb % 3 - 1) - P.ax.zi[0][vx0 + lx[j]]));
+                            const double d2 = P.a[1] * (f.y + (((nb / 3) % 3 - 1) - P.ax.zi[1][vy0 + ly[j]]));
+                            const double d3 = P.a[2] * (f.z + ((nb / 9 - 1) - P.ax.zi[2][vz0 + lz[j]]));
+                            in = ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
+                        }
+                        if (in) {
+                            const float w = __builtin_amdgcn_exp2f(dd * F.k2);
+                            // Kahan-compensated S += w * (cos, sin)
+                            const float yr = fmaf(w, pc.w, -cre[j]);
+                            const float tr = sre[j] + yr;
+                            cre[j] = (tr - sre[j]) - yr;
+                            sre[j] = tr;
+                            const float yi = fmaf(w, ssn[t], -cim[j]);
+                            const float ti = sim[j] + yi;
+                            cim[j] = (ti - sim[j]) - yi;
+                            sim[j] = ti;
+                            r2m[j] = fminf(r2m[j], dd);
+                            cnt[j]++;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < FAST_VPT; j++) {
+            if (!act[j]) continue;
+            const int ix = vx0 + lx[j], iy = vy0 + ly[j], iz = vz0 + lz[j];
+            const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
+            const float mag = sqrtf(fmaf(sre[j], sre[j], sim[j] * sim[j]));
+            const float ww = 1.0f / fmaxf(1.0e-16f, mag);
+            float rh = 1.0f;
+            if (P.use_density) {
+                const float s = (float)P.sigma;
+                const float r2 = r2m[j];
+                if (r2 < s * s) rh = 1.0f;
+                else if (r2 > 4 * s * s) rh = 0.0f;
+                else {
+                    const float xx = sqrtf(r2) / s - 1.0f;
+                    rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
+                }
+            }
+            field[o] = make_float4(sre[j] * ww, sim[j] * ww, rh, 0.0f);
+            if (out_cnt) out_cnt[o] = cnt[j];
+            if (out_r2m) out_r2m[o] = r2m[j];
+        }
+    }
+    // guard-band statistics: one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) nguard += __shfl_down(nguard, o, 64);
+    if ((tid & 63) == 0 && nguard) atomicAdd(guard_count, nguard);
+}
+
+// ------------------------------------------------------------------ (c) integrate, FAST
+
+struct FastIntParams {
+    int Mx, My, Mz, np;         // np: precession points in this pass
+    float b0, bg, dz;
+    float aa[32];
+};
+
+// One lane per pixel column, NP precession points marched together from
+// one read of the column (the field is shared by all precession points:
+// precession changes s_g only).  FP32 classic RK4 with h = 2 dz; the
+// per-step renormalisation of the reference is a positive scalar on a
+// linear map, so it is applied once at the end: I = |phi_g|^2/|phi|^2.
+template <int NP>
+__global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const float4* __restrict__ field,
+                                                        double* __restrict__ img, int accumulate,
+                                                        double inv_total)
+{
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)P.Mx * P.My;
+    if (col >= plane) return;
+    const int nz = P.Mz;
+    const float dz = P.dz, dz3 = P.dz / 3.0f, dz23 = 2.0f * P.dz / 3.0f, dz2 = 2.0f * P.dz;
+    float p0r[NP], p0i[NP], pgr[NP], pgi[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) { p0r[p] = 1.0f; p0i[p] = 0.0f; pgr[p] = 0.0f; pgi[p] = 0.0f; }
+    auto coef = [&](int k, float& c0, float& cgr, float& cgi) {
+        const float4 f = field[col + plane * (k == nz ? 0 : k)];
+        c0 = P.b0 * f.z;
+        const float bgr = P.bg * f.z;
+        cgr = bgr * f.x;
+        cgi = bgr * f.y;
+    };
+    float c0a = 0, cgra = 0, cgia = 0;
+    if (nz >= 2) coef(0, c0a, cgra, cgia);
+    for (int ii = 0; ii <= nz - 2; ii += 2) {
+        float c0b, cgrb, cgib, c0c, cgrc, cgic;
+        coef(ii + 1, c0b, cgrb, cgib);
+        coef(ii + 2, c0c, cgrc, cgic);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const float a = P.aa[p];
+            float d0r, d0i, dgr, dgi;
+#define FDERIV(c0, cgr, cgi)                                                   \
+    {                                                                          \
+        const float ca_ = c0 + a;                                              \
+        d0r = fmaf(-c0, p0i[p], fmaf(-cgi, pgr[p], -cgr * pgi[p]));           \
+        d0i = fmaf(c0, p0r[p], fmaf(cgr, pgr[p], -cgi * pgi[p]));             \
+        dgr = fmaf(cgi, p0r[p], fmaf(-cgr, p0i[p], -ca_ * pgi[p]));           \
+        dgi = fmaf(cgr, p0r[p], fmaf(cgi, p0i[p], ca_ * pgr[p]));             \
+    }
+            const float s0r = p0r[p], s0i = p0i[p], sgr = pgr[p], sgi = pgi[p];
+            FDERIV(c0a, cgra, cgia)
+            float a0r = fmaf(d0r, dz3, s0r), a0i = fmaf(d0i, dz3, s0i);
+            float agr = fmaf(dgr, dz3, sgr), agi = fmaf(dgi, dz3, sgi);
+            p0r[p] = fmaf(d0r, dz, s0r); p0i[p] = fmaf(d0i, dz, s0i);
+            pgr[p] = fmaf(dgr, dz, sgr); pgi[p] = fmaf(dgi, dz, sgi);
+            FDERIV(c0b, cgrb, cgib)
+            p0r[p] = fmaf(d0r, dz, s0r); p0i[p] = fmaf(d0i, dz, s0i);
+            pgr[p] = fmaf(dgr, dz, sgr); pgi[p] = fmaf(dgi, dz, sgi);
+            a0r = fmaf(d0r, dz23, a0r); a0i = fmaf(d0i, dz23, a0i);
+            agr = fmaf(dgr, dz23, agr); agi = fmaf(dgi, dz23, agi);
+            FDERIV(c0b, cgrb, cgib)
+            p0r[p] = fmaf(d0r, dz2, s0r); p0i[p] = fmaf(d0i, dz2, s0i);
+            pgr[p] = fmaf(dgr, dz2, sgr); pgi[p] = fmaf(dgi, dz2, sgi);
+            a0r = fmaf(d0r, dz23, a0r); a0i = fmaf(d0i, dz23, a0i);
+            agr = fmaf(dgr, dz23, agr); agi = fmaf(dgi, dz23, agi);
+            FDERIV(c0c, cgrc, cgic)
+#undef FDERIV
+            p0r[p] = fmaf(d0r, dz3, a0r); p0i[p] = fmaf(d0i, dz3, a0i);
+            pgr[p] = fmaf(dgr, dz3, agr); pgi[p] = fmaf(dgi, dz3, agi);
+        }
+        c0a = c0c; cgra = cgrc; cgia = cgic;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        if (p >= P.np) break;
+        const float ng = fmaf(pgr[p], pgr[p], pgi[p] * pgi[p]);
+        const float n0 = fmaf(p0r[p], p0r[p], p0i[p] * p0i[p]);
+        acc += (double)ng / (double)(n0 + ng);
+    }
+    acc *= inv_total;
+    img[col] = accumulate ? img[col] + acc : acc;
+}
+
+// ------------------------------------------------------------------ (d) blur
+
+struct BlurParams {
+    int nx, ny, nk;
+};
+
+static constexpr int BLUR_TILE = 256;
+static constexpr int BLUR_MAXK = 256;
+
+// x pass of gaussianBlurImage (d2bi.f90:1577-1599): row segments staged in
+// LDS with their halo; the kernel weights come from the host (exp in FP64).
+__global__ __launch_bounds__(BLUR_TILE) void k_blur_x(BlurParams B, const double* __restrict__ fin,
+                                                      const double* __restrict__ kern,
+                                                      double* __restrict__ tmp, double* __restrict__ wsx)
+{
+    __shared__ double row[BLUR_TILE + 2 * BLUR_MAXK];
+    const int iy = blockIdx.y;
+    const int x0 = blockIdx.x * BLUR_TILE;
+    const int lo = x0 - B.nk, n = BLUR_TILE + 2 * B.nk;
+    for (int t = threadIdx.x; t < n; t += BLUR_TILE) {
+        const int gx = lo + t;
+        row[t] = (gx >= 0 && gx < B.nx) ? fin[(int64_t)gx + (int64_t)B.nx * iy] : 0.0;
+    }
+    __syncthreads();
+    const int ix = x0 + threadIdx.x;
+    if (ix >= B.nx) return;
+    const int j0 = max(0, ix - B.nk), j1 = min(B.nx - 1, ix + B.nk);
+    double wf = 0.0, ws = 0.0;
+    for (int jj = j0; jj <= j1; jj++) {
+        const double ww = kern[abs(jj - ix)];
+        wf = wf + ww * row[jj - lo];
+        ws = ws + ww;
+    }
+    tmp[(int64_t)ix + (int64_t)B.nx * iy] = wf;
+    if (iy == 0) wsx[ix] = 1 / ws;
+}
+
+// y pass (d2bi.f90:1605-1619): out = sum_y k * tmp * (1/W_x) / W_y.
+__global__ __launch_bounds__(256) void k_blur_y(BlurParams B, const double* __restrict__ tmp,
+                                                const double* __restrict__ kern,
+                                                const double* __restrict__ wsx, double* __restrict__ out)
+{
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (ix >= B.nx || iy >= B.ny) return;
+    const int j0 = max(0, iy - B.nk), j1 = min(B.ny - 1, iy + B.nk);
+    double wf = 0.0, ws = 0.0;
+    for (int jj = j0; jj <= j1; jj++) {
+        const double ww = kern[abs(jj - iy)];
+        wf = wf + ww * tmp[(int64_t)ix + (int64_t)B.nx * jj];
+        ws = ws + ww;
+    }
+    out[(int64_t)ix + (int64_t)B.nx * iy] = (wf * wsx[ix]) / ws;
+}
+
+// ------------------------------------------------------------------ generic link-cell query
+
+struct QueryParams {
+    double a[9], ia[9];
+    int Nx, Ny, Nz;
+    double rc2;
+    int maxNeigh;
+};
+
+// neighbourList1b (Lib_LinkCell3D.f90:1515-1605) for arbitrary query points
+// and a general (not necessarily cuboidal) cell.
+__global__ void k_lc_query(QueryParams Q, int64_t nq, const double* __restrict__ q,
+                           const int* __restrict__ cell_start, const int* __restrict__ slot_id,
+                           const double4* __restrict__ frac, int* __restrict__ nout,
+                           int* __restrict__ idout, double* __restrict__ r2out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const double x[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    double zi[3];
+    for (int k = 0; k < 3; k++) zi[k] = (Q.ia[k] * x[0] + Q.ia[3 + k] * x[1]) + Q.ia[6 + k] * x[2];
+    const int ix = cell_of(floor(zi[0]), Q.Nx);
+    const int iy = cell_of(floor(zi[1]), Q.Ny);
+    const int iz = cell_of(floor(zi[2]), Q.Nz);
+    zi[0] = zi[0] - ix; zi[1] = zi[1] - iy; zi[2] = zi[2] - iz;
+    int n = 0;
+    for (int kz = -1; kz <= 1; kz++) {
+        const int jz = (iz + Q.Nz + kz) % Q.Nz;
+        const double dzi3 = kz - zi[2];
+        for (int ky = -1; ky <= 1; ky++) {
+            const int jy = (iy + Q.Ny + ky) % Q.Ny;
+            const double dzi2 = ky - zi[1];
+            for (int kx = -1; kx <= 1; kx++) {
+                const int jx = (ix + Q.Nx + kx) % Q.Nx;
+                const double dzi1 = kx - zi[0];
+                const int c = jx + Q.Nx * (jy + Q.Ny * jz);
+                for (int s = cell_start[c]; s < cell_start[c + 1]; s++) {
+                    const double4 f = frac[s];
+                    const double dzj1 = f.x + dzi1, dzj2 = f.y + dzi2, dzj3 = f.z + dzi3;
+                    double d[3];
+                    for (int k = 0; k < 3; k++) d[k] = (Q.a[k] * dzj1 + Q.a[3 + k] * dzj2) + Q.a[6 + k] * dzj3;
+                    const double dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+                    if (dd <= Q.rc2) {
+                        if (n < Q.maxNeigh) {
+                            idout[i * Q.maxNeigh + n] = slot_id[s];
+                            r2out[i * Q.maxNeigh + n] = dd;
+                        }
+                        n++;
+                    }
+                }
+            }
+        }
+    }
+    nout[i] = n;
+}
+
+// ------------------------------------------------------------------ orchestration
+
+struct Timer {
+    hipEvent_t* ev;
+    hipStream_t s;
+    int record(int i)
+    {
+        HIPCHK(hipEventRecord(ev[i], s));
+        return D2BI_OK;
+    }
+    double ms(int a, int b)
+    {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+        return t;
+    }
+};
+
+static void fill_vox_params(const d2bi_record* r, const d2bi_plan* p, const AxisDev& ax, VoxParams* V)
+{
+    V->Mx = p->Mx; V->My = p->My; V->Mz = p->Mz;
+    V->Ex = p->Ex; V->Ey = p->Ey; V->Ez = p->Ez;
+    V->a[0] = p->a_col[0]; V->a[1] = p->a_col[4]; V->a[2] = p->a_col[8];
+    V->rc2 = p->rc2; V->i2s2 = p->i2s2; V->sigma = p->sigma;
+    V->use_density = r->use_density ? 1 : 0;
+    V->ax = ax;
+}
+
+// Voxelise into the mode's field buffers; optional count / r2min outputs.
+static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables& tab, const AxisDev& ax,
+                     int mode, bool want_diag, int64_t* n_guard, hipStream_t s, Timer* T = nullptr)
+{
+    Workspace& w = g.ws;
+    const int64_t nvox = (int64_t)p->Mx * p->My * p->Mz;
+    VoxParams V;
+    fill_vox_params(r, p, ax, &V);
+    int* dcnt = nullptr;
+    void* dr2 = nullptr;
+    if (want_diag) {
+        CHK(w.cnt.ensure(sizeof(int) * nvox));
+        CHK(w.r2m.ensure(sizeof(double) * nvox));
+        dcnt = w.cnt.as<int>();
+        dr2 = w.r2m.p;
+    }
+    if (mode == D2BI_MODE_FP64) {
+        CHK(w.f64_re.ensure(sizeof(double) * nvox));
+        CHK(w.f64_im.ensure(sizeof(double) * nvox));
+        CHK(w.f64_rho.ensure(sizeof(double) * nvox));
+        dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
+        if (T) CHK(T->record(2));
+        hipLaunchKernelGGL(k_voxel_fp64, grid, dim3(256), 0, s, V, w.cell_start.as<int>(),
+                           w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), w.f64_re.as<double>(),
+                           w.f64_im.as<double>(), w.f64_rho.as<double>(), dcnt, (double*)dr2);
+        HIPCHK(hipGetLastError());
+        if (T) CHK(T->record(3));
+        return D2BI_OK;
+    }
+    CHK(w.f32_field.ensure(sizeof(float4) * nvox));
+    FastParams F;
+    F.v = V;
+    F.nax = tab.hi[0] - tab.lo[0] + 1;
+    F.nay = tab.hi[1] - tab.lo[1] + 1;
+    F.naz = tab.hi[2] - tab.lo[2] + 1;
+    F.nblocks = F.nax * F.nay * F.naz;
+    F.per_xcd = (F.nblocks + 7) / 8;
+    // guard band: FP32 positions carry <= ~3e-6 A of rounding; 1e-5 relative
+    // on (3 sigma)^2 covers it with margin
+    F.rc2_lo = (float)(p->rc2 * (1.0 - 1.0e-5));
+    F.rc2_hi = (float)(p->rc2 * (1.0 + 1.0e-5));
+    F.k2 = (float)(-p->i2s2 * 1.4426950408889634);
+    CHK(w.counters.ensure(64));
+    unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
+    HIPCHK(hipMemsetAsync(gc, 0, 8, s));
+    if (T) CHK(T->record(2));
+    hipLaunchKernelGGL(k_voxel_fast, dim3(8 * F.per_xcd), dim3(FAST_BLOCK), 0, s, F,
+                       w.cell_start.as<int>(), w.slot_frac.as<double4>(), w.slot_cs32.as<float2>(),
+                       w.f32_field.as<float4>(), dcnt, (float*)dr2, gc);
+    HIPCHK(hipGetLastError());
+    if (T) CHK(T->record(3));
+    if (n_guard) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, gc, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *n_guard = (int64_t)v;
+    }
+    return D2BI_OK;
+}
+
+template <int NP>
+static void launch_int_fast(const FastIntParams& P, const float4* field, double* img, int acc, double inv,
+                            hipStream_t s)
+{
+    const int64_t plane = (int64_t)P.Mx * P.My;
+    hipLaunchKernelGGL(k_integrate_fast<NP>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
+                       field, img, acc, inv);
+}
+
+static int run_integrate(const d2bi_plan* p, int mode, hipStream_t s, Timer* T = nullptr)
+{
+    Workspace& w = g.ws;
+    const int64_t plane = (int64_t)p->Mx * p->My;
+    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
+    double* d_img = w.img.as<double>();
+    if (mode == D2BI_MODE_FP64) {
+        double* d_aa = d_img + plane;
+        HIPCHK(hipMemcpyAsync(d_aa, p->aa, sizeof(double) * p->nPrec, hipMemcpyHostToDevice, s));
+        IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa};
+        if (T) CHK(T->record(4));
+        hipLaunchKernelGGL(k_integrate_fp64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
+                           w.f64_re.as<double>(), w.f64_im.as<double>(), w.f64_rho.as<double>(), d_img);
+        HIPCHK(hipGetLastError());
+        if (T) CHK(T->record(5));
+        // the aa copy reads p->aa (caller memory): finish before returning
+        HIPCHK(hipStreamSynchronize(s));
+        return D2BI_OK;
+    }
+    const double inv = 1.0 / p->nPrec;
+    if (T) CHK(T->record(4));
+    for (int p0 = 0, first = 1; p0 < p->nPrec; first = 0) {
+        const int left = p->nPrec - p0;
+        const int np = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+        FastIntParams P;
+        P.Mx = p->Mx; P.My = p->My; P.Mz = p->Mz; P.np = np;
+        P.b0 = (float)p->b0; P.bg = (float)p->bg; P.dz = (float)p->dz;
+        for (int i = 0; i < 32; i++) P.aa[i] = i < np ? (float)p->aa[p0 + i] : 0.0f;
+        const float4* f = w.f32_field.as<float4>();
+        switch (np) {
+            case 32: launch_int_fast<32>(P, f, d_img, !first, inv, s); break;
+            case 16: launch_int_fast<16>(P, f, d_img, !first, inv, s); break;
+            case 8: launch_int_fast<8>(P, f, d_img, !first, inv, s); break;
+            case 4: launch_int_fast<4>(P, f, d_img, !first, inv, s); break;
+            case 2: launch_int_fast<2>(P, f, d_img, !first, inv, s); break;
+            default: launch_int_fast<1>(P, f, d_img, !first, inv, s); break;
+        }
+        HIPCHK(hipGetLastError());
+        p0 += np;
+    }
+    if (T) CHK(T->record(5));
+    return D2BI_OK;
+}
+
+// Whole condition: bin -> voxelise -> integrate, result left in ws.img.
+static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
+                           int mode, d2bi_stats* st, d2bi_plan* plan_out)
+{
+    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
+    d2bi_plan plan;
+    CHK(plan_image(r, n, precAngle, nPrec, &plan));
+    AxisTables tab;
+    CHK(axis_tables(r, &plan, &tab));
+    BinParams BP;
+    CHK(bin_params(r, &plan, &BP));
+    hipStream_t s = g.stream;
+    Timer T{g.ev, s};
+    CHK(T.record(0));
+    AxisDev ax;
+    CHK(upload_axis(tab, &ax, s));
+    Binned b;
+    CHK(run_binning(d_x, n, BP, &b, s));
+    CHK(T.record(1));
+    int64_t ng = 0;
+    CHK(run_voxel(r, &plan, tab, ax, mode, false, st ? &ng : nullptr, s, &T));
+    CHK(run_integrate(&plan, mode, s, &T));
+    CHK(T.record(6));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        int mpc = 0;
+        HIPCHK(hipMemcpy(&mpc, g.ws.counters.p, sizeof(int), hipMemcpyDeviceToHost));
+        st->n_points += b.nr;
+        st->n_guard += ng;
+        st->max_per_cell = std::max(st->max_per_cell, mpc);
+        st->n_cells = b.ncell;
+        st->ms_bin += T.ms(0, 1);
+        st->ms_voxel += T.ms(2, 3);          // the voxel kernel alone
+        st->ms_integrate += T.ms(4, 5);      // the integration kernels alone
+        st->ms_total += T.ms(0, 6);
+    }
+    if (plan_out) *plan_out = plan;
+    return D2BI_OK;
+}
+
+static int upload_atoms_tmp(const double* x, int64_t ld, int64_t n, double** d_x)
+{
+    if (n < 0 || (n > 0 && (!x || ld < 3))) return seterr(D2BI_ERR_ARG, "bad atom array");
+    CHK(g.ws.x.ensure(sizeof(double) * 3 * std::max<int64_t>(n, 1)));
+    void* hp;
+    CHK(pinned(sizeof(double) * 3 * std::max<int64_t>(n, 1), &hp));
+    double* h = (double*)hp;
+    for (int64_t j = 0; j < n; j++) {
+        h[3 * j] = x[ld * j]; h[3 * j + 1] = x[ld * j + 1]; h[3 * j + 2] = x[ld * j + 2];
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(g.ws.x.p, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    *d_x = g.ws.x.as<double>();
+    return D2BI_OK;
+}
+
+}  // namespace d2bi
+
+// ------------------------------------------------------------------ C-ABI
+
+using namespace d2bi;
+
+extern "C" {
+
+int d2bi_version(void) { return 10000; }
+
+const char* d2bi_last_error(void) { return g_err; }
+
+int d2bi_gpu_init(int device)
+{
+    if (g.init) {
+        if (device == g.device) return D2BI_OK;
+        d2bi_gpu_finalize();
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return seterr(D2BI_ERR_NODEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return seterr(D2BI_ERR_ARG, "device %d out of range (%d visible)", device, n);
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    for (auto& e : g.ev) HIPCHK(hipEventCreate(&e));
+    g.device = device;
+    g.init = true;
+    return D2BI_OK;
+}
+
+int d2bi_gpu_finalize(void)
+{
+    if (!g.init) return D2BI_OK;
+    (void)hipSetDevice(g.device);
+    (void)hipStreamSynchronize(g.stream);
+    g.ws.release_all();
+    for (auto& e : g.ev) { (void)hipEventDestroy(e); e = nullptr; }
+    (void)hipStreamDestroy(g.stream);
+    g.stream = nullptr;
+    g.init = false;
+    g.device = -1;
+    return D2BI_OK;
+}
+
+int d2bi_gpu_atoms_upload(const double* x, int64_t ld, int64_t nAtoms, d2bi_atoms* out)
+{
+    CHK(need_init());
+    if (!out || nAtoms < 0 || (nAtoms > 0 && (!x || ld < 3))) return seterr(D2BI_ERR_ARG, "bad atom array");
+    d2bi_atoms_s* a = new d2bi_atoms_s;
+    a->n = nAtoms;
+    size_t bytes = sizeof(double) * 3 * std::max<int64_t>(nAtoms, 1);
+    if (hipMalloc(&a->x, bytes) != hipSuccess) {
+        delete a;
+        return seterr(D2BI_ERR_HIP, "hipMalloc for atoms failed");
+    }
+    std::vector<double> h(3 * std::max<int64_t>(nAtoms, 1));
+    for (int64_t j = 0; j < nAtoms; j++) {
+        h[3 * j] = x[ld * j]; h[3 * j + 1] = x[ld * j + 1]; h[3 * j + 2] = x[ld * j + 2];
+    }
+    if (hipMemcpy(a->x, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(a->x);
+        delete a;
+        return seterr(D2BI_ERR_HIP, "atom upload failed");
+    }
+    *out = a;
+    return D2BI_OK;
+}
+
+int d2bi_gpu_atoms_free(d2bi_atoms atoms)
+{
+    if (!atoms) return D2BI_OK;
+    if (atoms->x) (void)hipFree(atoms->x);
+    delete atoms;
+    return D2BI_OK;
+}
+
+int64_t d2bi_gpu_atoms_count(d2bi_atoms atoms) { return atoms ? atoms->n : -1; }
+
+int d2bi_gpu_image_dev(const d2bi_record* rec, d2bi_atoms atoms, double precAngle, int32_t nPrec, int32_t mode,
+                       double* img, d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (!rec || !atoms || !img) return seterr(D2BI_ERR_ARG, "null argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    d2bi_plan plan;
+    CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
+    HIPCHK(hipMemcpy(img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToHost));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_image(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms, double precAngle,
+                   int32_t nPrec, int32_t mode, double* img, d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (!rec || !img) return seterr(D2BI_ERR_ARG, "null argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    double* d_x;
+    CHK(upload_atoms_tmp(x, ld, nAtoms, &d_x));
+    d2bi_plan plan;
+    CHK(image_condition(rec, d_x, nAtoms, precAngle, nPrec, mode, stats, &plan));
+    HIPCHK(hipMemcpy(img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToHost));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_image_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int32_t nRec, const double* precAngle,
+                         const int32_t* nPrec, int32_t mode, double* img, d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (!recs || !atoms || !precAngle || !nPrec || !img || nRec < 0) return seterr(D2BI_ERR_ARG, "null argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    int64_t off = 0;
+    for (int i = 0; i < nRec; i++) {
+        if (!atoms[i]) return seterr(D2BI_ERR_ARG, "null atom set %d", i);
+        d2bi_plan plan;
+        CHK(image_condition(&recs[i], atoms[i]->x, atoms[i]->n, precAngle[i], nPrec[i], mode, stats, &plan));
+        const int64_t plane = (int64_t)plan.Mx * plan.My;
+        HIPCHK(hipMemcpy(img + off, g.ws.img.p, sizeof(double) * plane, hipMemcpyDeviceToHost));
+        off += plane;
+    }
+    return D2BI_OK;
+}
+
+int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms, int32_t mode,
+                   double* phase_re_im, double* rho, int32_t* count, double* r2min)
+{
+    CHK(need_init());
+    if (!rec) return seterr(D2BI_ERR_ARG, "null argument");
+    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
+    double* d_x;
+    CHK(upload_atoms_tmp(x, ld, nAtoms, &d_x));
+    d2bi_plan plan;
+    CHK(plan_image(rec, nAtoms, 0.0, 1, &plan));
+    AxisTables tab;
+    CHK(axis_tables(rec, &plan, &tab));
+    BinParams BP;
+    CHK(bin_params(rec, &plan, &BP));
+    hipStream_t s = g.stream;
+    AxisDev ax;
+    CHK(upload_axis(tab, &ax, s));
+    Binned b;
+    CHK(run_binning(d_x, nAtoms, BP, &b, s));
+    CHK(run_voxel(rec, &plan, tab, ax, mode, true, nullptr, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int64_t nvox = (int64_t)plan.Mx * plan.My * plan.Mz;
+    Workspace& w = g.ws;
+    if (mode == D2BI_MODE_FP64) {
+        if (phase_re_im) {
+            std::vector<double> re(nvox), im(nvox);
+            HIPCHK(hipMemcpy(re.data(), w.f64_re.p, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(im.data(), w.f64_im.p, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < nvox; i++) { phase_re_im[2 * i] = re[i]; phase_re_im[2 * i + 1] = im[i]; }
+        }
+        if (rho) HIPCHK(hipMemcpy(rho, w.f64_rho.p, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+        if (r2min) HIPCHK(hipMemcpy(r2min, w.r2m.p, sizeof(double) * nvox, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(4 * nvox);
+        HIPCHK(hipMemcpy(f.data(), w.f32_field.p, sizeof(float) * 4 * nvox, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nvox; i++) {
+            if (phase_re_im) { phase_re_im[2 * i] = f[4 * i]; phase_re_im[2 * i + 1] = f[4 * i + 1]; }
+            if (rho) rho[i] = f[4 * i + 2];
+        }
+        if (r2min) {
+            std::vector<float> r(nvox);
+            HIPCHK(hipMemcpy(r.data(), w.r2m.p, sizeof(float) * nvox, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < nvox; i++) r2min[i] = r[i];
+        }
+    }
+    if (count) HIPCHK(hipMemcpy(count, w.cnt.p, sizeof(int) * nvox, hipMemcpyDeviceToHost));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_linkcells(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms, int32_t* cell_start,
+                       int32_t* ids, double* frac, double* gdotr, int64_t cap, int64_t* nPoints)
+{
+    CHK(need_init());
+    if (!rec) return seterr(D2BI_ERR_ARG, "null argument");
+    double* d_x;
+    CHK(upload_atoms_tmp(x, ld, nAtoms, &d_x));
+    d2bi_plan plan;
+    CHK(plan_image(rec, nAtoms, 0.0, 1, &plan));
+    BinParams BP;
+    CHK(bin_params(rec, &plan, &BP));
+    Binned b;
+    CHK(run_binning(d_x, nAtoms, BP, &b, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (nPoints) *nPoints = b.nr;
+    if ((ids || frac || gdotr) && b.nr > cap)
+        return seterr(D2BI_ERR_CAPACITY, "%lld points exceed cap %lld", (long long)b.nr, (long long)cap);
+    Workspace& w = g.ws;
+    if (cell_start) HIPCHK(hipMemcpy(cell_start, w.cell_start.p, sizeof(int) * (b.ncell + 1), hipMemcpyDeviceToHost));
+    if (ids && b.nr) HIPCHK(hipMemcpy(ids, w.slot_id.p, sizeof(int) * b.nr, hipMemcpyDeviceToHost));
+    if (frac && b.nr) {
+        std::vector<double> f(4 * b.nr);
+        HIPCHK(hipMemcpy(f.data(), w.slot_frac.p, sizeof(double) * 4 * b.nr, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < b.nr; i++) { frac[3 * i] = f[4 * i]; frac[3 * i + 1] = f[4 * i + 1]; frac[3 * i + 2] = f[4 * i + 2]; }
+    }
+    if (gdotr && b.nr) HIPCHK(hipMemcpy(gdotr, w.pt_gdotr.p, sizeof(double) * b.nr, hipMemcpyDeviceToHost));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_lc_query(const double a[9], int32_t Nx, int32_t Ny, int32_t Nz, const double* p, int64_t nP,
+                      const double* q, int64_t nQ, double rc, int32_t maxNeigh, int32_t* n, int32_t* id, double* r2)
+{
+    CHK(need_init());
+    if (!a || Nx < 1 || Ny < 1 || Nz < 1 || nP < 0 || nQ < 0 || maxNeigh < 0 || (nQ > 0 && (!q || !n)) ||
+        (nP > 0 && !p) || (maxNeigh > 0 && nQ > 0 && (!id || !r2)))
+        return seterr(D2BI_ERR_ARG, "bad argument");
+    hipStream_t s = g.stream;
+    double* d_x;
+    CHK(upload_atoms_tmp(p, 3, nP, &d_x));
+    BinParams BP;
+    std::memset(&BP, 0, sizeof BP);
+    double ia[9];
+    inverse3(a, ia);
+    std::memcpy(BP.ia, ia, sizeof ia);
+    BP.nuvw = 1;
+    BP.Ex = Nx; BP.Ey = Ny; BP.Ez = Nz;
+    BP.generic = 1;
+    Binned b;
+    CHK(run_binning(d_x, nP, BP, &b, s));
+    QueryParams Q;
+    std::memcpy(Q.a, a, sizeof Q.a);
+    std::memcpy(Q.ia, ia, sizeof Q.ia);
+    Q.Nx = Nx; Q.Ny = Ny; Q.Nz = Nz;
+    Q.rc2 = rc * rc;
+    Q.maxNeigh = maxNeigh;
+    Workspace& w = g.ws;
+    if (nQ > 0) {
+        Buf dq, dn, did, dr2;
+        int rc_ = D2BI_OK;
+        if ((rc_ = dq.ensure(sizeof(double) * 3 * nQ)) || (rc_ = dn.ensure(sizeof(int) * nQ)) ||
+            (rc_ = did.ensure(sizeof(int) * std::max<int64_t>(1, nQ * maxNeigh))) ||
+            (rc_ = dr2.ensure(sizeof(double) * std::max<int64_t>(1, nQ * maxNeigh)))) {
+            dq.release(); dn.release(); did.release(); dr2.release();
+            return rc_;
+        }
+        hipError_t e = hipMemcpy(dq.p, q, sizeof(double) * 3 * nQ, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+        hipLaunchKernelGGL(k_lc_query, dim3((unsigned)((nQ + 255) / 256)), dim3(256), 0, s, Q, nQ, dq.as<double>(),
+                           w.cell_start.as<int>(), w.slot_id.as<int>(), w.slot_frac.as<double4>(), dn.as<int>(),
+                           did.as<int>(), dr2.as<double>());
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = hipMemcpy(n, dn.p, sizeof(int) * nQ, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && maxNeigh > 0) e = hipMemcpy(id, did.p, sizeof(int) * nQ * maxNeigh, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && maxNeigh > 0) e = hipMemcpy(r2, dr2.p, sizeof(double) * nQ * maxNeigh, hipMemcpyDeviceToHost);
+        dq.release(); dn.release(); did.release(); dr2.release();
+        if (e != hipSuccess) return seterr(D2BI_ERR_HIP, "link-cell query failed: %s", hipGetErrorString(e));
+    }
+    return D2BI_OK;
+}
+
+int d2bi_gpu_blur(const double* in, int32_t nx, int32_t ny, double t, double* out)
+{
+    CHK(need_init());
+    if (!in || !out || nx < 1 || ny < 1 || !(t > 0)) return seterr(D2BI_ERR_ARG, "bad blur argument");
+    int nk = (int)std::ceil(t * 5);
+    if (nk < 5) nk = 5;
+    if (nk > BLUR_MAXK) return seterr(D2BI_ERR_CAPACITY, "blur half-width %d exceeds %d", nk, BLUR_MAXK);
+    std::vector<double> kern(nk + 1);
+    const double i2s2 = 1 / (2 * t * t);
+    kern[0] = 1.0;
+    for (int i = 1; i <= nk; i++) kern[i] = std::exp((double)(-i * i) * i2s2);
+    Workspace& w = g.ws;
+    const int64_t np = (int64_t)nx * ny;
+    CHK(w.blur_in.ensure(sizeof(double) * np));
+    CHK(w.blur_tmp.ensure(sizeof(double) * np));
+    CHK(w.blur_out.ensure(sizeof(double) * np));
+    CHK(w.blur_w.ensure(sizeof(double) * nx));
+    CHK(w.blur_k.ensure(sizeof(double) * (nk + 1)));
+    hipStream_t s = g.stream;
+    HIPCHK(hipMemcpyAsync(w.blur_in.p, in, sizeof(double) * np, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.blur_k.p, kern.data(), sizeof(double) * (nk + 1), hipMemcpyHostToDevice, s));
+    BlurParams B{nx, ny, nk};
+    hipLaunchKernelGGL(k_blur_x, dim3((nx + BLUR_TILE - 1) / BLUR_TILE, ny), dim3(BLUR_TILE), 0, s, B,
+                       w.blur_in.as<double>(), w.blur_k.as<double>(), w.blur_tmp.as<double>(), w.blur_w.as<double>());
+    hipLaunchKernelGGL(k_blur_y, dim3((nx + 63) / 64, (ny + 3) / 4), dim3(256), 0, s, B, w.blur_tmp.as<double>(),
+                       w.blur_k.as<double>(), w.blur_w.as<double>(), w.blur_out.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, w.blur_out.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return D2BI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+"""Extract the reference's own test data into small fixtures.
+
+Run in the build container only (it reads the reference checkout as text):
+
+    python tests/golden/make_fixtures.py /root/reference
+
+Writes, next to this script:
+  linkcell_kat.json     the 200 positions and expected outputs of
+                        test/testLib_LinkCell3D.f90:66-290
+  w_loop_golden.npz     Data/test/test.i111_loop_W.lammps atom positions and
+                        box, and the golden image + header of
+                        Data/test/...g2_0_0.k0.00_0.00_1.00_ng1.00.dat
+The fixtures are data (inputs and expected outputs); no reference source
+text is stored.
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def linkcell_kat(ref):
+    src = open(os.path.join(ref, "test", "testLib_LinkCell3D.f90")).read()
+    # positions: the reshape( (/ ... /),(/ 3,200 /) ) literal
+    m = re.search(r"xx\s*=\s*reshape\(\s*\(/(.*?)/\)\s*,\s*\(/\s*3\s*,\s*200\s*/\)", src, re.S)
+    nums = [float(v) for v in re.findall(r"([-+]?\d+\.\d+)d0", m.group(1))]
+    assert len(nums) == 600, len(nums)
+    pos = np.array(nums).reshape(200, 3)
+    # expected output lines: the output0 array of strings
+    m = re.search(r"output0\s*=\s*\(/(.*?)/\)", src, re.S)
+    lines = [s.strip() for s in re.findall(r'"([^"]*)"', m.group(1))]
+    assert len(lines) == 20, len(lines)
+    neigh = []
+    for ln in lines[6:19]:
+        t = ln.split()
+        # neigh <n> id <id> dx <dx1> <dx2> <dx3> |dx| <d>
+        neigh.append({"id": int(t[3]), "dx": [float(t[5]), float(t[6]), float(t[7])],
+                      "d": float(t[9])})
+    exc_self = [int(v) for v in lines[19].split()[3:]]
+    return {
+        "source": "test/testLib_LinkCell3D.f90:66-290",
+        "a_cell": [[1, 0, 0], [0, 1, 0], [0, 0, 1]],
+        "N": [6, 4, 5],
+        "positions": pos.tolist(),
+        "n_atoms": int(lines[0].split()[-1]),
+        "max_count": int(lines[1].split()[-1]),
+        "max_storage": int(lines[2].split()[-1]),
+        "max_neigh": int(lines[3].split()[-1]),
+        "atom100": [float(v) for v in lines[4].split()[3:6]],
+        "nearest": {"x": [float(v) for v in lines[5].split()[2:5]],
+                    "dx": [float(v) for v in lines[5].split()[6:9]],
+                    "d": float(lines[5].split()[10])},
+        "neigh_r1": neigh,
+        "neigh_exc_self": exc_self,
+        "expected_lines": lines,
+    }
+
+
+def w_loop(ref):
+    lam = os.path.join(ref, "Data", "test", "test.i111_loop_W.lammps")
+    lines = open(lam).read().splitlines()
+    n_atoms = int(lines[2].split()[0])
+    box = []
+    for ln in lines[:12]:
+        for ax in ("xlo", "ylo", "zlo"):
+            if ax in ln:
+                lo, hi = (float(v) for v in ln.split()[:2])
+                box.append((lo, hi))
+    start = next(i for i, ln in enumerate(lines) if ln.startswith("Atoms")) + 2
+    xyz = np.array([[float(v) for v in lines[start + i].split()[2:5]] for i in range(n_atoms)])
+    ids = np.array([int(lines[start + i].split()[0]) for i in range(n_atoms)], np.int32)
+    dat = os.path.join(ref, "Data", "test", "test.i111_loop_W.lammps.g2_0_0.k0.00_0.00_1.00_ng1.00.dat")
+    dl = open(dat).read().splitlines()
+    hdr = {}
+    for ln in dl[:11]:
+        key = ln[1:].split()[0] if ln.startswith("#") else None
+        hdr[ln] = key
+    g_line = dl[2].split()
+    xi = [float(v) for v in dl[5].split()[3:5]]
+    sg = float(dl[7].split()[-1])
+    space = [float(v) for v in dl[8].split()[-3:]]
+    scale = float(dl[9].split()[-1])
+    rot = [float(v) for v in dl[10].split()[2:11]]
+    nx, ny = (int(v) for v in dl[11].split())
+    vals = []
+    for ln in dl[12:]:
+        vals.extend(float(ln[i:i + 12]) for i in range(0, len(ln), 12) if ln[i:i + 12].strip())
+    img = np.array(vals).reshape(ny, nx)        # written x fastest: row = iy
+    return dict(
+        x=xyz, ids=ids, box=np.array(box), modg=float(g_line[-1]), xi=np.array(xi), sg=sg,
+        space=np.array(space), scale=scale, rotation_cm=np.array(rot), image=img,
+        header="\n".join(dl[:11]))
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    with open(os.path.join(HERE, "linkcell_kat.json"), "w") as f:
+        json.dump(linkcell_kat(ref), f, indent=1)
+    w = w_loop(ref)
+    np.savez_compressed(os.path.join(HERE, "w_loop_golden.npz"), **w)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,261 @@
+"""HIP path vs the FP64 oracle, through the C-ABI.
+
+Bars (per the imaging contract):
+  link cells, point ids, in-cell offsets, g.r   bit-exact
+  voxel membership (neighbour count), r2min     bit-exact in FP64 mode,
+                                                identical counts in FAST mode
+  FP64 image                                    max |dI| <= 1e-10
+  FAST image                                    max |dI| <= 1e-4
+  blur                                          max |d| <= 1e-15
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from simulated_tem_amd import imaging, synth
+from simulated_tem_amd.imaging import DynamicalTwoBeamImaging, input_supercell, lattice_a0
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FAST_TOL = 1e-4
+NTH = min(16, os.cpu_count() or 1)
+
+
+def _cond(name, cells=None, sg=None, density=None, g=None, d=None, k=None):
+    cfg = synth.scaled(name, cells, d)
+    if sg is not None:
+        cfg["sg"] = [sg]
+    if density is not None:
+        cfg["density"] = density
+    if g is not None:
+        cfg["g"] = [g]
+    if k is not None:
+        cfg["k"] = k
+    cfg["n_tomo"] = 1
+    x, box = synth.make_atoms(cfg)
+    x, conds = synth.conditions(cfg, x, box)
+    return x, conds[0][0]
+
+
+CASES = {
+    "c0_sg0": lambda: _cond("c0"),
+    "c0_sg0.01_density": lambda: _cond("c0", sg=0.01, density=True),
+    "c1_small_zpad": lambda: _cond("c1", cells=(22, 18, 12)),
+    "c2_small_g200_density": lambda: _cond("c2", cells=(14, 12, 30), sg=-0.02),
+    "c0_k111_g1m10": lambda: _cond("c0", cells=(9, 11, 7), g=(1, -1, 0), k=(1, 1, 1),
+                                   d=(25.0, 27.0, 19.0)),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request):
+    print("\n[case]", request.param, flush=True)
+    x, im = CASES[request.param]()
+    oc = orc.Condition(im.rec, x)
+    return request.param, x, im, oc
+
+
+def test_linkcells_bitexact(case):
+    name, x, im, oc = case
+    cs, ids, frac, gd = im.linkcells(x)
+    ocs, oids, ofrac = oc.cells()
+    assert len(ids) == oc.n_points
+    assert np.array_equal(cs, ocs)
+    assert np.array_equal(ids, oids)
+    assert np.array_equal(frac, ofrac)
+    assert np.array_equal(gd, oc.gdotr())
+    # every point in exactly one cell, offsets in [0,1)
+    assert np.array_equal(np.sort(ids), np.arange(1, len(ids) + 1))
+    assert np.all(frac >= 0) and np.all(frac < 1)
+
+
+def test_field_fp64_membership_bitexact(case):
+    name, x, im, oc = case
+    ph, rho, cnt, r2m = im.field(x, "fp64")
+    oph, orho, ocnt, or2m = oc.field(nthreads=NTH)
+    assert np.array_equal(cnt, ocnt)
+    assert np.array_equal(r2m, or2m)
+    assert np.max(np.abs(ph - oph)) < 1e-12
+    assert np.max(np.abs(rho - orho)) < 1e-14
+
+
+def test_field_fast_membership_exact(case):
+    name, x, im, oc = case
+    ph, rho, cnt, r2m = im.field(x, "fast")
+    oph, orho, ocnt, or2m = oc.field(nthreads=NTH)
+    assert np.array_equal(cnt, ocnt)                      # guard band keeps membership exact
+    occ = ocnt > 0
+    assert np.max(np.abs(r2m[occ] - or2m[occ]) / np.maximum(or2m[occ], 1.0)) < 1e-5
+    err = np.abs(ph - oph)
+    print("[fast field] %s max|dx| %.3e mean %.3e" % (name, err.max(), err.mean()), flush=True)
+    assert np.max(np.abs(rho - orho)) < 1e-5
+
+
+@pytest.mark.parametrize("n_prec", [1, 5])
+def test_image_fp64_and_fast(case, n_prec):
+    name, x, im, oc = case
+    ref = orc.Condition(im.rec, x, 0.005, n_prec).image(nthreads=NTH)
+    g64 = im.computeD2BI_img(x, 0.005, n_prec, mode="fp64")
+    gf = im.computeD2BI_img(x, 0.005, n_prec, mode="fast")
+    e64 = np.max(np.abs(g64 - ref))
+    ef = np.abs(gf - ref)
+    print("[image] %s P=%d fp64 max %.3e  fast max %.3e mean %.3e  I in [%.4f, %.4f]"
+          % (name, n_prec, e64, ef.max(), ef.mean(), ref.min(), ref.max()), flush=True)
+    assert e64 <= FP64_TOL
+    assert ef.max() <= FAST_TOL
+
+
+def test_blur_matches_oracle():
+    rng = np.random.default_rng(7)
+    for (ny, nx, t) in [(94, 94, 3.1446), (800, 800, 3.16), (33, 517, 0.4), (300, 7, 12.3)]:
+        img = rng.uniform(size=(ny, nx))
+        got = imaging.gaussianBlurImage(img, t)
+        want = orc.blur(img, t)
+        assert np.max(np.abs(got - want)) <= 1e-15, (ny, nx, t)
+
+
+def _kat():
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden", "linkcell_kat.json")))
+
+
+def test_linkcell_kat_gpu():
+    """testLib_LinkCell3D.f90: atom 100's neighbours within r = 1, in order."""
+    k = _kat()
+    pos = np.array(k["positions"])
+    lc = imaging.LinkCell3D(np.eye(3), *k["N"])
+    lc.add(pos)
+    olc = orc.LinkCells(np.eye(3), *k["N"], orc.estimate_max_count(120, 200))
+    for i, p in enumerate(pos):
+        olc.add(i + 1, p)
+    cs, ids, frac = olc.flatten(200)
+    slot = int(np.where(ids == 100)[0][0])
+    c = int(np.searchsorted(cs, slot, side="right") - 1)
+    cz, rem = divmod(c, 24)
+    cy, cx = divmod(rem, 6)
+    y100 = frac[slot] + np.array([cx, cy, cz])
+    (gid, gr2), = lc.neighbourList([y100], 1.0)
+    assert list(gid) == [e["id"] for e in k["neigh_r1"]]
+    for e, rr in zip(k["neigh_r1"], gr2):
+        assert abs(math.sqrt(rr) - e["d"]) <= 5.1e-9
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_linkcell_query_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    N = rng.integers(3, 8, size=3)
+    a = np.diag(rng.uniform(0.9, 1.6, size=3))
+    a[0, 1] = 0.1                                        # a sheared cell, general 3x3 path
+    rc = 0.8
+    box = a @ np.diag(N)
+    pts = rng.uniform(0, 1, size=(2000, 3)) @ box.T
+    q = rng.uniform(0, 1, size=(300, 3)) @ box.T
+    lc = imaging.LinkCell3D(a, *N)
+    lc.add(pts)
+    got = lc.neighbourList(q, rc)
+    olc = orc.LinkCells(a, *N, 8)
+    for i, p in enumerate(pts):
+        olc.add(i + 1, p)
+    for qq, (gid, gr2) in zip(q, got):
+        oid, or2 = olc.neighbours(qq, rc)
+        assert np.array_equal(gid, oid)
+        assert np.array_equal(gr2, or2)
+
+
+def test_empty_and_single_atom():
+    cfg = synth.scaled("c0", cells=(6, 6, 6))
+    x, box = synth.make_atoms(cfg)
+    _, conds = synth.conditions(cfg, x, box)
+    im = conds[0][0]
+    img = im.computeD2BI_img(np.zeros((0, 3)), mode="fp64")
+    assert np.all(img == 0.0)
+    img = im.computeD2BI_img(np.zeros((0, 3)), mode="fast")
+    assert np.all(img == 0.0)
+    # one atom: phase factor e^{i g.r} on every voxel within 3 sigma
+    one = np.array([[9.0, 9.5, 10.0]])
+    ph, rho, cnt, r2m = im.field(one, "fp64")
+    oc = orc.Condition(im.rec, one)
+    gd = oc.gdotr()
+    assert oc.n_points >= 1
+    hit = cnt > 0
+    assert np.any(hit)
+    assert np.allclose(ph[hit], np.exp(1j * gd[0]), atol=1e-13) or len(gd) > 1
+    assert np.all(ph[~hit] == 0)
+    img64 = im.computeD2BI_img(one, mode="fp64")
+    ref = oc.image()
+    assert np.max(np.abs(img64 - ref)) <= FP64_TOL
+
+
+def test_batch_equals_single():
+    x, im1 = _cond("c0", cells=(10, 10, 10))
+    _, im2 = _cond("c0", cells=(10, 10, 10), sg=0.015, density=True)
+    at = imaging.Atoms(x)
+    out = imaging.computeD2BI_img_batch([(im1, at, 0.005, 3), (im2, at, 0.005, 1)], mode="fast")
+    assert np.array_equal(out[0], im1.computeD2BI_img(at, 0.005, 3, mode="fast"))
+    assert np.array_equal(out[1], im2.computeD2BI_img(at, 0.005, 1, mode="fast"))
+
+
+def test_golden_w_loop_image(golden_dir):
+    """Data/test/...ng1.00.dat via -U <header RU> -R I -d <header extents>: the
+    header holds RU to 6 decimals and the extents to 3, so the reproduction
+    is approximate; testD2BI.sh's min/max/avg gate is +-1/256."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    a0 = lattice_a0(3.1652)
+    box = w["box"][:, 1] - w["box"][:, 0]
+    sa, sn = input_supercell(np.diag(box), 4, a0)
+    im = DynamicalTwoBeamImaging(sa, sn, a0, (0, 0, 1), (2, 0, 0), 200.0, 4, 4)
+    im.setConventionalUnitCell(a0 * np.eye(3))
+    im.setVoxelSupercellSize(w["space"])
+    im.setSupercellTilt(w["rotation_cm"].reshape(3, 3).T, np.eye(3))
+    im.setExtinctionDistances(*w["xi"])
+    im.setUseDensityField(False)
+    x = w["x"]
+    g64 = im.computeD2BI_img(x, mode="fp64")
+    gf = im.computeD2BI_img(x, mode="fast")
+    ref = orc.Condition(im.rec, x).image(nthreads=NTH)
+    assert np.max(np.abs(g64 - ref)) <= FP64_TOL
+    assert np.max(np.abs(gf - ref)) <= FAST_TOL
+    gold = w["image"]
+    d = np.abs(g64 - gold)
+    print("[golden] max|d| %.3e mean|d| %.3e  min/max/avg %.6f %.6f %.6f"
+          % (d.max(), d.mean(), g64.min(), g64.max(), g64.mean()), flush=True)
+    for got, want in zip((g64.min(), g64.max(), g64.mean()), (0.000028, 0.230127, 0.081501)):
+        assert abs(got - want) <= 0.0039
+
+
+@pytest.fixture(scope="module")
+def c1_full():
+    print("\n[c1 full] generating 8M atoms", flush=True)
+    cfg = dict(synth.CONFIGS["c1"])
+    x, box = synth.make_atoms(cfg)
+    x, conds = synth.conditions(cfg, x, box)
+    im = conds[0][0]
+    at = imaging.Atoms(x)
+    return x, im, at
+
+
+def test_c1_full_size_window_parity(c1_full):
+    """Full 800x800x400 image on the GPU; the oracle recomputes a window of
+    columns (with the whole crystal binned) for the 1e-10 / 1e-4 bars."""
+    x, im, at = c1_full
+    g64 = im.computeD2BI_img(at, mode="fp64")
+    gf = im.computeD2BI_img(at, mode="fast")
+    assert g64.shape == (800, 800)
+    print("[c1 full] oracle binning", flush=True)
+    oc = orc.Condition(im.rec, x)
+    for (ix0, iy0) in [(0, 0), (392, 397), (792, 790)]:
+        ref = oc.image(window=(ix0, ix0 + 8, iy0, iy0 + 10), nthreads=NTH)
+        e64 = np.max(np.abs(g64[iy0:iy0 + 10, ix0:ix0 + 8] - ref))
+        ef = np.max(np.abs(gf[iy0:iy0 + 10, ix0:ix0 + 8] - ref))
+        print("[c1 full] window (%d,%d) fp64 %.3e fast %.3e" % (ix0, iy0, e64, ef), flush=True)
+        assert e64 <= FP64_TOL
+        assert ef <= FAST_TOL
+    # full-image error gate FAST vs FP64 (size-independent property)
+    e = np.abs(gf - g64)
+    print("[c1 full] fast vs fp64: max %.3e mean %.3e" % (e.max(), e.mean()), flush=True)
+    assert e.max() <= FAST_TOL
+    assert np.all(g64 >= 0) and np.all(g64 <= 1)
