@@ -110,15 +110,20 @@ def main():
     log("[bench] fast: %.3f ms/step, %.3e imaged voxels/s" % (ms_step, value))
 
     # plain per-condition FP64 path, same condition
-    st64 = D2biStats()
-    img64 = step("fp64", st64)
-    t0 = time.perf_counter()
-    for _ in range(args.fp64_steps):
+    fp64_value, err, ms64 = None, None, None
+    if args.fp64_steps > 0:
+        st64 = D2biStats()
         img64 = step("fp64", st64)
-    t1 = time.perf_counter()
-    fp64_value = imaged * args.fp64_steps / (t1 - t0)
-    err = np.abs(img_fast - img64)
-    log("[bench] fp64: %.3e imaged voxels/s, fast vs fp64 max %.3e" % (fp64_value, err.max()))
+        ms64 = []
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            st64 = D2biStats()
+            img64 = step("fp64", st64)
+            ms64.append((st64.ms_total, st64.ms_voxel))
+        t1 = time.perf_counter()
+        fp64_value = imaged * args.fp64_steps / (t1 - t0)
+        err = np.abs(img_fast - img64)
+        log("[bench] fp64: %.3e imaged voxels/s, fast vs fp64 max %.3e" % (fp64_value, err.max()))
 
     t_vox = float(np.mean(ms_vox)) * 1e-3
     achieved = vox * FLOPS_PER_VOXEL / t_vox / 1e12
@@ -135,11 +140,14 @@ def main():
         "data": "synthetic (seeded bcc crystal, Eshelby centres, voids)",
         "config": {"workload": args.workload, "atoms": len(x), "voxels": [p.Mx, p.My, p.Mz],
                    "n_prec": n_prec, "g": list(cfg["g"][0]), "sg": cfg["sg"][0]},
-        "fp64_value": fp64_value, "fast_over_fp64": value / fp64_value,
-        "max_abs_err_vs_fp64": float(err.max()), "mean_abs_err_vs_fp64": float(err.mean()),
+        "fp64_value": fp64_value, "fast_over_fp64": value / fp64_value if fp64_value else None,
+        "max_abs_err_vs_fp64": float(err.max()) if err is not None else None,
+        "mean_abs_err_vs_fp64": float(err.mean()) if err is not None else None,
         "ms_bin": float(np.mean(ms_bin)), "ms_voxel": float(np.mean(ms_vox)),
-        "ms_integrate": float(np.mean(ms_int)), "ms_fp64_total": st64.ms_total,
-        "ms_fp64_voxel": st64.ms_voxel, "n_guard": int(st.n_guard),
+        "ms_integrate": float(np.mean(ms_int)),
+        "ms_fp64_total": float(np.mean([m[0] for m in ms64])) if ms64 else None,
+        "ms_fp64_voxel": float(np.mean([m[1] for m in ms64])) if ms64 else None,
+        "n_guard": int(st.n_guard),
         "roofline": roof,
     }
     if not args.no_cpu:

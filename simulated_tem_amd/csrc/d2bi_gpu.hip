@@ -89,7 +89,7 @@ struct Workspace {
     Buf mask, blk_cnt, blk_off;
     Buf pt_cell, pt_slot, pt_frac, pt_gdotr;
     Buf cell_cnt, cell_start, cell_ids;
-    Buf slot_id, slot_frac, slot_cs64, slot_cs32;
+    Buf slot_id, slot_frac, slot_cs64, slot_p32, slot_sn32;
     Buf scan_tmp;
     Buf counters;               // misc device counters (guard count, max per cell)
     // voxel axis tables
@@ -106,7 +106,7 @@ struct Workspace {
     {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
-                      &slot_cs32, &scan_tmp, &counters, &ax_cell, &ax_zi, &ax_start, &f64_re,
+                      &slot_p32, &slot_sn32, &scan_tmp, &counters, &ax_cell, &ax_zi, &ax_start, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_k};
         for (Buf* b : all) b->release();
@@ -271,6 +271,7 @@ struct BinParams {
     int Ex, Ey, Ez;
     int64_t n;                  // atoms
     int generic;                // 1: LinkCell3D add1 (no transform, no filter)
+    double cell_a[3];           // diagonal of the link-cell matrix (FAST-mode records)
 };
 
 // Link-cell index of a reduced coordinate: mod(floor(y) + 16*N, N)
@@ -418,9 +419,10 @@ __global__ void k_bin_place(int64_t nr, const int* __restrict__ pt_cell, const i
 // id, FP64 offset in cell, FP64 and FP32 (cos, sin) of g.r.
 __global__ void k_bin_finish(int ncell, const int* __restrict__ cell_start, int* __restrict__ cell_ids,
                              const double* __restrict__ pt_frac, const double* __restrict__ pt_gdotr,
+                             double a0, double a1, double a2,
                              int* __restrict__ slot_id, double4* __restrict__ slot_frac,
-                             double2* __restrict__ slot_cs64, float2* __restrict__ slot_cs32,
-                             int* __restrict__ max_per_cell)
+                             double2* __restrict__ slot_cs64, float4* __restrict__ slot_p32,
+                             float* __restrict__ slot_sn32, int* __restrict__ max_per_cell)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
@@ -439,7 +441,11 @@ __global__ void k_bin_finish(int ncell, const int* __restrict__ cell_start, int*
         double sn, cs;
         sincos(pt_gdotr[ii], &sn, &cs);
         slot_cs64[s] = make_double2(cs, sn);
-        slot_cs32[s] = make_float2((float)cs, (float)sn);
+        // FAST-mode record: offset from the cell origin in A (cuboidal cells)
+        // and cos(g.r), all rounded from FP64; sin(g.r) beside it
+        slot_p32[s] = make_float4((float)(a0 * pt_frac[3 * (int64_t)ii]), (float)(a1 * pt_frac[3 * (int64_t)ii + 1]),
+                                  (float)(a2 * pt_frac[3 * (int64_t)ii + 2]), (float)cs);
+        slot_sn32[s] = (float)sn;
     }
     if (s1 - s0 > 0) atomicMax(max_per_cell, s1 - s0);
 }
@@ -490,7 +496,8 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     CHK(w.slot_id.ensure(sizeof(int) * nrc));
     CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
     CHK(w.slot_cs64.ensure(sizeof(double2) * nrc));
-    CHK(w.slot_cs32.ensure(sizeof(float2) * nrc));
+    CHK(w.slot_p32.ensure(sizeof(float4) * nrc));
+    CHK(w.slot_sn32.ensure(sizeof(float) * nrc));
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
                            w.mask.as<uint32_t>(), w.blk_off.as<int>(), nblk, w.pt_cell.as<int>(),
@@ -507,8 +514,9 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     }
     hipLaunchKernelGGL(k_bin_finish, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell,
                        w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
-                       w.pt_gdotr.as<double>(), w.slot_id.as<int>(), w.slot_frac.as<double4>(),
-                       w.slot_cs64.as<double2>(), w.slot_cs32.as<float2>(), w.counters.as<int>());
+                       w.pt_gdotr.as<double>(), P.cell_a[0], P.cell_a[1], P.cell_a[2], w.slot_id.as<int>(),
+                       w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), w.slot_p32.as<float4>(),
+                       w.slot_sn32.as<float>(), w.counters.as<int>());
     HIPCHK(hipGetLastError());
     out->nr = nr;
     out->ncell = ncell;
@@ -538,6 +546,7 @@ static int bin_params(const d2bi_record* r, const d2bi_plan* p, BinParams* P)
     P->nuvw = r->nuvw;
     P->Ex = p->Ex; P->Ey = p->Ey; P->Ez = p->Ez;
     P->generic = 0;
+    P->cell_a[0] = p->a_col[0]; P->cell_a[1] = p->a_col[4]; P->cell_a[2] = p->a_col[8];
     return D2BI_OK;
 }
 
@@ -793,170 +802,233 @@ __global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const doubl
 
 // ------------------------------------------------------------------ (b) voxelise, FAST
 
-static constexpr int FAST_BLOCK = 256;
-static constexpr int FAST_VPT = 2;                    // voxels per thread per pass
-static constexpr int FAST_CHUNK = 512;                // staged candidates per pass
+static constexpr int TILE = 4;                       // 4x4x4 voxels = one wave64
+static constexpr int TILE_CAP = 192;                 // staged candidates per pass (LDS ~5.4 KB/wave)
+static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
 
 struct FastParams {
     VoxParams v;
-    int nax, nay, naz;          // active link cells per axis
-    int nblocks;                // nax*nay*naz
-    int per_xcd;                // blocks per XCD slice (XCD-aware swizzle)
+    int ntx, nty, ntz;          // tiles per axis
+    int ntiles, per_xcd;
     float rc2_lo, rc2_hi;       // guard band around (3 sigma)^2
+    float rc_cull2;             // (3 sigma)^2 widened for the tile-box cull
     float k2;                   // -log2(e)/(2 sigma^2)
+    float cellf[3];             // link-cell sides in FP32
 };
 
-// One workgroup per link cell: every voxel of the cell shares the 27-cell
-// neighbourhood and its wrap offsets.  Candidates (positions relative to the
-// cell origin, FP64 -> FP32 after adding the neighbour offset, and FP32
-// cos/sin of g.r computed in FP64) are staged in LDS and read by all lanes
-// as broadcasts.  Candidates whose FP32 squared range falls within the
-// guard band of (3 sigma)^2 are re-tested with the FP64 expression of
-// neighbourList1b, so membership is exact.  S is Kahan-compensated.
-__global__ __launch_bounds__(FAST_BLOCK) void k_voxel_fast(FastParams F, const int* __restrict__ cell_start,
-                                                           const double4* __restrict__ frac,
-                                                           const float2* __restrict__ cs32,
-                                                           float4* __restrict__ field,
-                                                           int* __restrict__ out_cnt,
-                                                           float* __restrict__ out_r2m,
-                                                           unsigned long long* __restrict__ guard_count)
+__device__ inline float wave_min(float v)
 {
-    // XCD-aware order: hardware dispatches consecutive workgroups round-robin
-    // over the 8 XCDs; give each XCD a contiguous slab of link cells so that
-    // neighbouring cells (which share 18 of 27 neighbour cells) hit one L2.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One wave per 4x4x4 voxel tile.  The tile's candidates are the points of
+// every link cell in the unwrapped range [cell(first voxel)-1,
+// cell(last voxel)+1] on each axis; an unwrapped cell u stands for cell
+// mod(u, E) displaced by u - mod(u, E) cells, which is exactly the periodic
+// image neighbourList1b sees through its mod() (Lib_LinkCell3D.f90:1567-1575).
+// Candidates farther than 3 sigma from the tile's voxel box are dropped,
+// the rest are compacted into LDS (ballot + popcount) and every lane (voxel)
+// tests them all.  Position of a candidate relative to the tile's origin
+// cell: a*(frac + u - c0), computed in FP64 then rounded to FP32; of a voxel:
+// a*(zi + cell - c0).  Candidates two cells away from a voxel are always
+// beyond 3 sigma (link cells are >= 3 sigma wide); in the guard band the FP64
+// re-test also requires the neighbour offset to be -1, 0 or +1, as the
+// 27-cell loop does.  S is Kahan-compensated in FP32; cos/sin of g.r were
+// evaluated in FP64 during binning.
+template <bool DENSITY, bool DIAG>
+__global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
+                                                   const double4* __restrict__ frac,
+                                                   const float4* __restrict__ p32,
+                                                   const float* __restrict__ sn32,
+                                                   float4* __restrict__ field,
+                                                   int* __restrict__ out_cnt,
+                                                   float* __restrict__ out_r2m,
+                                                   unsigned long long* __restrict__ guard_count)
+{
+    // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
+    // give each XCD a contiguous range of tiles so neighbouring tiles, which
+    // read the same link cells, share one L2.
     const int bid = blockIdx.x;
     const int L = (bid & 7) * F.per_xcd + (bid >> 3);
-    if (L >= F.nblocks) return;
+    if (L >= F.ntiles) return;
     const VoxParams& P = F.v;
-    const int bx = L % F.nax, by = (L / F.nax) % F.nay, bz = L / (F.nax * F.nay);
-    const int cx = P.ax.cell[0][0] + bx, cy = P.ax.cell[1][0] + by, cz = P.ax.cell[2][0] + bz;
-    const int vx0 = P.ax.start[0][cx], vx1 = P.ax.start[0][cx + 1];
-    const int vy0 = P.ax.start[1][cy], vy1 = P.ax.start[1][cy + 1];
-    const int vz0 = P.ax.start[2][cz], vz1 = P.ax.start[2][cz + 1];
-    const int nvx = vx1 - vx0, nvy = vy1 - vy0, nvz = vz1 - vz0;
-    const int nvox = nvx * nvy * nvz;
-    if (nvox <= 0) return;
+    const int tx = L % F.ntx, ty = (L / F.ntx) % F.nty, tz = L / (F.ntx * F.nty);
+    const int lane = threadIdx.x;
+    const int x0 = tx * TILE, y0 = ty * TILE, z0 = tz * TILE;
+    const int x1 = min(P.Mx, x0 + TILE) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TILE) - 1;
+    const int ix = x0 + (lane & 3), iy = y0 + ((lane >> 2) & 3), iz = z0 + (lane >> 4);
+    const bool act = ix <= x1 && iy <= y1 && iz <= z1;
+    const int jx = min(ix, x1), jy = min(iy, y1), jz = min(iz, z1);
 
-    __shared__ int nb_start[27], nb_off[28];
-    __shared__ float4 sp[FAST_CHUNK];          // x, y, z (A, rel. cell origin), cos
-    __shared__ float ssn[FAST_CHUNK];          // sin
-    __shared__ int sslot[FAST_CHUNK];          // slot of the candidate
-    __shared__ unsigned char snb[FAST_CHUNK];  // its neighbour-cell index 0..26
-    const int tid = threadIdx.x;
-    if (tid < 27) {
-        const int kx = tid % 3 - 1, ky = (tid / 3) % 3 - 1, kz = tid / 9 - 1;
-        const int jx = (cx + P.Ex + kx) % P.Ex, jy = (cy + P.Ey + ky) % P.Ey, jz = (cz + P.Ez + kz) % P.Ez;
-        const int c = jx + P.Ex * (jy + P.Ey * jz);
-        nb_start[tid] = cell_start[c];
-        nb_off[tid + 1] = cell_start[c + 1] - cell_start[c];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        nb_off[0] = 0;
-        for (int i = 1; i <= 27; i++) nb_off[i] += nb_off[i - 1];
-    }
-    __syncthreads();
-    const int T = nb_off[27];
-    unsigned long long nguard = 0;
+    // origin cell of the tile and the voxel's own cell / offset
+    const int c0x = P.ax.cell[0][x0], c0y = P.ax.cell[1][y0], c0z = P.ax.cell[2][z0];
+    const int lcx = P.ax.cell[0][jx] - c0x, lcy = P.ax.cell[1][jy] - c0y, lcz = P.ax.cell[2][jz] - c0z;
+    const double zix = P.ax.zi[0][jx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
+    const float qx = (float)(P.a[0] * (zix + lcx));
+    const float qy = (float)(P.a[1] * (ziy + lcy));
+    const float qz = (float)(P.a[2] * (ziz + lcz));
+    const float bx0 = wave_min(qx), bx1 = wave_max(qx);
+    const float by0 = wave_min(qy), by1 = wave_max(qy);
+    const float bz0 = wave_min(qz), bz1 = wave_max(qz);
 
-    for (int vbase = 0; vbase < nvox; vbase += FAST_BLOCK * FAST_VPT) {
-        float qx[FAST_VPT], qy[FAST_VPT], qz[FAST_VPT];
-        float sre[FAST_VPT], sim[FAST_VPT], cre[FAST_VPT], cim[FAST_VPT], r2m[FAST_VPT];
-        int cnt[FAST_VPT], lx[FAST_VPT], ly[FAST_VPT], lz[FAST_VPT];
-        bool act[FAST_VPT];
-#pragma unroll
-        for (int j = 0; j < FAST_VPT; j++) {
-            const int v = vbase + tid + j * FAST_BLOCK;
-            act[j] = v < nvox;
-            const int vv = act[j] ? v : 0;
-            lx[j] = vv % nvx; ly[j] = (vv / nvx) % nvy; lz[j] = vv / (nvx * nvy);
-            qx[j] = (float)(P.a[0] * P.ax.zi[0][vx0 + lx[j]]);
-            qy[j] = (float)(P.a[1] * P.ax.zi[1][vy0 + ly[j]]);
-            qz[j] = (float)(P.a[2] * P.ax.zi[2][vz0 + lz[j]]);
-            sre[j] = sim[j] = cre[j] = cim[j] = 0.0f;
-            r2m[j] = 3.4028234663852886e38f;
-            cnt[j] = 0;
+    // unwrapped cell range
+    const int ux0 = c0x - 1, uy0 = c0y - 1, uz0 = c0z - 1;
+    const int ncx = P.ax.cell[0][x1] + 1 - ux0 + 1;
+    const int ncy = P.ax.cell[1][y1] + 1 - uy0 + 1;
+    const int ncz = P.ax.cell[2][z1] + 1 - uz0 + 1;
+    const int ncell = ncx * ncy * ncz;
+
+    __shared__ int c_beg[TILE_MAXC];
+    __shared__ int c_off[TILE_MAXC + 1];
+    __shared__ float4 sp[TILE_CAP];            // x, y, z (A, rel. origin cell), cos
+    __shared__ float ssn[TILE_CAP];            // sin
+    __shared__ int sslot[TILE_CAP];            // slot
+    __shared__ int su[TILE_CAP];               // unwrapped cell offsets (dx + 8 dy + 64 dz)
+
+    // per-cell point ranges and their prefix sum (one cell per lane per round)
+    int carry = 0;
+    for (int cb = 0; cb < ncell; cb += 64) {
+        const int c = cb + lane;
+        int cnt = 0;
+        if (c < ncell) {
+            const int dx = c % ncx, dy = (c / ncx) % ncy, dz = c / (ncx * ncy);
+            const int rx = (ux0 + dx + P.Ex) % P.Ex, ry = (uy0 + dy + P.Ey) % P.Ey, rz = (uz0 + dz + P.Ez) % P.Ez;
+            const int rc = rx + P.Ex * (ry + P.Ey * rz);
+            const int b = cell_start[rc];
+            cnt = cell_start[rc + 1] - b;
+            c_beg[c] = b;
         }
-        for (int base = 0; base < T; base += FAST_CHUNK) {
-            const int nch = min(FAST_CHUNK, T - base);
-            __syncthreads();
-            for (int t = tid; t < nch; t += FAST_BLOCK) {
-                const int gi = base + t;
-                int nb = 0;
-                while (nb_off[nb + 1] <= gi) nb++;
-                const int slot = nb_start[nb] + (gi - nb_off[nb]);
-                const int kx = nb % 3 - 1, ky = (nb / 3) % 3 - 1, kz = nb / 9 - 1;
-                const double4 f = frac[slot];
-                const float2 c = cs32[slot];
-                sp[t] = make_float4((float)(P.a[0] * (f.x + kx)), (float)(P.a[1] * (f.y + ky)),
-                                    (float)(P.a[2] * (f.z + kz)), c.x);
-                ssn[t] = c.y;
-                sslot[t] = slot;
-                snb[t] = (unsigned char)nb;
-            }
-            __syncthreads();
-            for (int t = 0; t < nch; t++) {
-                const float4 pc = sp[t];
+        int incl = cnt;
 #pragma unroll
-                for (int j = 0; j < FAST_VPT; j++) {
-                    const float dx = pc.x - qx[j], dy = pc.y - qy[j], dz = pc.z - qz[j];
-                    const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (act[j] && dd <= F.rc2_hi) {
-                        bool in = true;
-                        if (dd >= F.rc2_lo) {
-                            // exact FP64 test, neighbourList1b (Lib_LinkCell3D.f90:1579-1596)
-                            nguard++;
-                            const int sl = sslot[t], nb = snb[t];
-                            const double4 f = frac[sl];
-                            const double d1 = P.a[0] * (f.x + ((nb % 3 - 1) - P.ax.zi[0][vx0 + lx[j]]));
-                            const double d2 = P.a[1] * (f.y + (((nb / 3) % 3 - 1) - P.ax.zi[1][vy0 + ly[j]]));
-                            const double d3 = P.a[2] * (f.z + ((nb / 9 - 1) - P.ax.zi[2][vz0 + lz[j]]));
-                            in = ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
-                        }
-                        if (in) {
-                            const float w = __builtin_amdgcn_exp2f(dd * F.k2);
-                            // Kahan-compensated S += w * (cos, sin)
-                            const float yr = fmaf(w, pc.w, -cre[j]);
-                            const float tr = sre[j] + yr;
-                            cre[j] = (tr - sre[j]) - yr;
-                            sre[j] = tr;
-                            const float yi = fmaf(w, ssn[t], -cim[j]);
-                            const float ti = sim[j] + yi;
-                            cim[j] = (ti - sim[j]) - yi;
-                            sim[j] = ti;
-                            r2m[j] = fminf(r2m[j], dd);
-                            cnt[j]++;
-                        }
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (c < ncell) c_off[c] = carry + incl - cnt;
+        carry += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) c_off[ncell] = carry;
+    __syncthreads();
+    const int T = carry;
+
+    const float k2 = F.k2;
+    float sre = 0.0f, sim = 0.0f, cre = 0.0f, cim = 0.0f;
+    float r2m = 3.4028234663852886e38f;
+    int cnt = 0;
+    unsigned long long nguard = 0;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    int t_next = 0;
+    while (t_next < T) {
+        // stage: compact candidates within 3 sigma of the tile box
+        int n = 0;
+        while (t_next < T && n <= TILE_CAP - 64) {
+            const int t = t_next + lane;
+            bool keep = false;
+            float4 pc;
+            float sn = 0.0f;
+            int slot = 0, upk = 0;
+            if (t < T) {
+                int lo = 0, hi = ncell - 1;          // cell holding candidate t
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (c_off[mid] <= t) lo = mid; else hi = mid - 1;
+                }
+                const int c = lo;
+                slot = c_beg[c] + (t - c_off[c]);
+                const int dx = c % ncx, dy = (c / ncx) % ncy, dz = c / (ncx * ncy);
+                upk = dx + 8 * dy + 64 * dz;
+                pc = p32[slot];
+                pc.x += F.cellf[0] * (float)(dx - 1);
+                pc.y += F.cellf[1] * (float)(dy - 1);
+                pc.z += F.cellf[2] * (float)(dz - 1);
+                const float ex = fmaxf(0.0f, fmaxf(bx0 - pc.x, pc.x - bx1));
+                const float ey = fmaxf(0.0f, fmaxf(by0 - pc.y, pc.y - by1));
+                const float ez = fmaxf(0.0f, fmaxf(bz0 - pc.z, pc.z - bz1));
+                keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= F.rc_cull2;
+                if (keep) sn = sn32[slot];
+            }
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const int pos = n + (int)__popcll(m & lt);
+                sp[pos] = pc;
+                ssn[pos] = sn;
+                sslot[pos] = slot;
+                su[pos] = upk;
+            }
+            n += (int)__popcll(m);
+            t_next += 64;
+        }
+        __syncthreads();
+        // every voxel against every staged candidate (LDS broadcasts)
+        for (int j = 0; j < n; j++) {
+            const float4 pc = sp[j];
+            const float dx = pc.x - qx, dy = pc.y - qy, dz = pc.z - qz;
+            const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            bool in = act && dd <= F.rc2_lo;
+            const bool band = act && dd > F.rc2_lo && dd <= F.rc2_hi;
+            if (__ballot(band)) {
+                if (band) {
+                    // exact FP64 test of neighbourList1b (Lib_LinkCell3D.f90:1565-1596)
+                    nguard++;
+                    const int u = su[j];
+                    const int kx = (u & 7) - 1 - lcx, ky = ((u >> 3) & 7) - 1 - lcy, kz = (u >> 6) - 1 - lcz;
+                    if (kx >= -1 && kx <= 1 && ky >= -1 && ky <= 1 && kz >= -1 && kz <= 1) {
+                        const double4 f = frac[sslot[j]];
+                        const double d1 = P.a[0] * (f.x + (kx - zix));
+                        const double d2 = P.a[1] * (f.y + (ky - ziy));
+                        const double d3 = P.a[2] * (f.z + (kz - ziz));
+                        in = ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
                     }
                 }
             }
+            const float w = in ? __builtin_amdgcn_exp2f(dd * k2) : 0.0f;
+            // Kahan-compensated S += w (cos, sin)
+            const float yr = fmaf(w, pc.w, -cre);
+            const float tr = sre + yr;
+            cre = (tr - sre) - yr;
+            sre = tr;
+            const float yi = fmaf(w, ssn[j], -cim);
+            const float ti = sim + yi;
+            cim = (ti - sim) - yi;
+            sim = ti;
+            if (DENSITY || DIAG) r2m = in ? fminf(r2m, dd) : r2m;
+            if (DIAG) cnt += in ? 1 : 0;
         }
-#pragma unroll
-        for (int j = 0; j < FAST_VPT; j++) {
-            if (!act[j]) continue;
-            const int ix = vx0 + lx[j], iy = vy0 + ly[j], iz = vz0 + lz[j];
-            const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
-            const float mag = sqrtf(fmaf(sre[j], sre[j], sim[j] * sim[j]));
-            const float ww = 1.0f / fmaxf(1.0e-16f, mag);
-            float rh = 1.0f;
-            if (P.use_density) {
-                const float s = (float)P.sigma;
-                const float r2 = r2m[j];
-                if (r2 < s * s) rh = 1.0f;
-                else if (r2 > 4 * s * s) rh = 0.0f;
-                else {
-                    const float xx = sqrtf(r2) / s - 1.0f;
-                    rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
-                }
+        __syncthreads();
+    }
+
+    if (act) {
+        const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
+        const float mag = sqrtf(fmaf(sre, sre, sim * sim));
+        const float ww = 1.0f / fmaxf(1.0e-16f, mag);
+        float rh = 1.0f;
+        if (DENSITY && P.use_density) {
+            // convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994) in FP32
+            const float s = (float)P.sigma;
+            if (r2m < s * s) rh = 1.0f;
+            else if (r2m > 4 * s * s) rh = 0.0f;
+            else {
+                const float xx = sqrtf(r2m) / s - 1.0f;
+                rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
             }
-            field[o] = make_float4(sre[j] * ww, sim[j] * ww, rh, 0.0f);
-            if (out_cnt) out_cnt[o] = cnt[j];
-            if (out_r2m) out_r2m[o] = r2m[j];
+        }
+        field[o] = make_float4(sre * ww, sim * ww, rh, 0.0f);
+        if (DIAG) {
+            out_cnt[o] = cnt;
+            out_r2m[o] = r2m;
         }
     }
-    // guard-band statistics: one atomic per wave
     for (int o = 32; o > 0; o >>= 1) nguard += __shfl_down(nguard, o, 64);
-    if ((tid & 63) == 0 && nguard) atomicAdd(guard_count, nguard);
+    if (lane == 0 && nguard) atomicAdd(guard_count, nguard);
 }
 
 // ------------------------------------------------------------------ (c) integrate, FAST
@@ -1216,23 +1288,45 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     CHK(w.f32_field.ensure(sizeof(float4) * nvox));
     FastParams F;
     F.v = V;
-    F.nax = tab.hi[0] - tab.lo[0] + 1;
-    F.nay = tab.hi[1] - tab.lo[1] + 1;
-    F.naz = tab.hi[2] - tab.lo[2] + 1;
-    F.nblocks = F.nax * F.nay * F.naz;
-    F.per_xcd = (F.nblocks + 7) / 8;
+    F.ntx = (p->Mx + TILE - 1) / TILE;
+    F.nty = (p->My + TILE - 1) / TILE;
+    F.ntz = (p->Mz + TILE - 1) / TILE;
+    F.ntiles = F.ntx * F.nty * F.ntz;
+    F.per_xcd = (F.ntiles + 7) / 8;
     // guard band: FP32 positions carry <= ~3e-6 A of rounding; 1e-5 relative
     // on (3 sigma)^2 covers it with margin
     F.rc2_lo = (float)(p->rc2 * (1.0 - 1.0e-5));
     F.rc2_hi = (float)(p->rc2 * (1.0 + 1.0e-5));
+    F.rc_cull2 = (float)(p->rc2 * (1.0 + 1.0e-4));
     F.k2 = (float)(-p->i2s2 * 1.4426950408889634);
+    F.cellf[0] = (float)p->a_col[0]; F.cellf[1] = (float)p->a_col[4]; F.cellf[2] = (float)p->a_col[8];
+    // the unwrapped cell range of a tile must fit the per-tile cell table
+    {
+        int worst = 1;
+        for (int a = 0; a < 3; a++) {
+            const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
+            int mx = 0;
+            for (int t0 = 0; t0 < M; t0 += TILE)
+                mx = std::max(mx, tab.cell[a][std::min(M, t0 + TILE) - 1] - tab.cell[a][t0] + 3);
+            worst *= mx;
+        }
+        if (worst > TILE_MAXC)
+            return seterr(D2BI_ERR_CAPACITY, "voxel tile spans %d link cells (max %d)", worst, TILE_MAXC);
+    }
     CHK(w.counters.ensure(64));
     unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
     HIPCHK(hipMemsetAsync(gc, 0, 8, s));
     if (T) CHK(T->record(2));
-    hipLaunchKernelGGL(k_voxel_fast, dim3(8 * F.per_xcd), dim3(FAST_BLOCK), 0, s, F,
-                       w.cell_start.as<int>(), w.slot_frac.as<double4>(), w.slot_cs32.as<float2>(),
-                       w.f32_field.as<float4>(), dcnt, (float*)dr2, gc);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, w.cell_start.as<int>(),
+                           w.slot_frac.as<double4>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>(),
+                           w.f32_field.as<float4>(), dcnt,
+                           (float*)dr2, gc);
+    };
+    const bool dens = r->use_density != 0;
+    if (want_diag) launch(k_voxel_tile<true, true>);
+    else if (dens) launch(k_voxel_tile<true, false>);
+    else launch(k_voxel_tile<false, false>);
     HIPCHK(hipGetLastError());
     if (T) CHK(T->record(3));
     if (n_guard) {
@@ -1570,6 +1664,7 @@ int d2bi_gpu_lc_query(const double a[9], int32_t Nx, int32_t Ny, int32_t Nz, con
     BP.nuvw = 1;
     BP.Ex = Nx; BP.Ey = Ny; BP.Ez = Nz;
     BP.generic = 1;
+    BP.cell_a[0] = a[0]; BP.cell_a[1] = a[4]; BP.cell_a[2] = a[8];
     Binned b;
     CHK(run_binning(d_x, nP, BP, &b, s));
     QueryParams Q;

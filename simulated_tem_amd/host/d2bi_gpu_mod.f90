@@ -1,0 +1,158 @@
+!   d2bi_gpu_mod.f90 -- iso_c_binding view of the imaging library's C-ABI
+!   (include/d2bi_gpu.h).  The derived types are bind(c) mirrors of the C
+!   structs; every interface is one entry point of libd2bi_gpu.so.
+module d2bi_gpu_mod
+    use iso_c_binding
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: D2BI_MODE_FP64 = 0
+    integer(c_int), parameter, public :: D2BI_MODE_FAST = 1
+    integer, parameter, public        :: D2BI_MAX_UVW = 125
+    integer, parameter, public        :: D2BI_MAX_PREC = 256
+
+    !   type DynamicalTwoBeamImaging fields read by computeD2BI_img
+    !   (Lib_DynamicalTwoBeamImaging.f90:64-90)
+    type, bind(c), public :: d2bi_record
+        real(c_double)     :: a0
+        real(c_double)     :: a_cell(9)
+        real(c_double)     :: g(3)
+        real(c_double)     :: E
+        real(c_double)     :: xi0, xig
+        real(c_double)     :: U(9), R(9)
+        real(c_double)     :: delta(3)
+        real(c_double)     :: super_a(9)
+        integer(c_int32_t) :: super_n(3)
+        integer(c_int32_t) :: nuvw
+        integer(c_int32_t) :: uvw(3*D2BI_MAX_UVW)
+        real(c_double)     :: gk_a(9)
+        integer(c_int32_t) :: gk_n(3)
+        integer(c_int32_t) :: use_density
+        real(c_double)     :: sg
+    end type d2bi_record
+
+    !   computeD2BI_img prologue (Lib_DynamicalTwoBeamImaging.f90:1635-1836)
+    type, bind(c), public :: d2bi_plan
+        integer(c_int32_t) :: Mx, My, Mz, deltaM, nMax, Ex, Ey, Ez, nPrec, pad0
+        real(c_double)     :: sigma, i2s2, rc2
+        real(c_double)     :: a_col(9), ia_col(9), lc_delta(3), RU(9), gg(3), xmax(3)
+        real(c_double)     :: aperpx, modg, dz, b0, bg
+        real(c_double)     :: aa(D2BI_MAX_PREC)
+    end type d2bi_plan
+
+    type, bind(c), public :: d2bi_stats
+        integer(c_int64_t) :: n_points, n_guard
+        integer(c_int32_t) :: max_per_cell, n_cells
+        real(c_double)     :: ms_bin, ms_voxel, ms_integrate, ms_total
+    end type d2bi_stats
+
+    public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_plan_image
+    public :: d2bi_setup_tilt_sg, d2bi_setup_tilt_ur, d2bi_rotate_to_standard_frame
+    public :: d2bi_gpu_atoms_upload, d2bi_gpu_atoms_free, d2bi_gpu_image_dev, d2bi_gpu_blur
+    public :: d2bi_write_png16, d2bi_error_message
+
+    interface
+        integer(c_int) function d2bi_gpu_init(device) bind(c, name="d2bi_gpu_init")
+            import :: c_int
+            integer(c_int), value :: device
+        end function
+        integer(c_int) function d2bi_gpu_finalize() bind(c, name="d2bi_gpu_finalize")
+            import :: c_int
+        end function
+        type(c_ptr) function d2bi_last_error() bind(c, name="d2bi_last_error")
+            import :: c_ptr
+        end function
+        integer(c_int) function d2bi_plan_image(rec, nAtoms, precAngle, nPrec, plan) &
+                bind(c, name="d2bi_plan_image")
+            import :: c_int, c_int32_t, c_int64_t, c_double, d2bi_record, d2bi_plan
+            type(d2bi_record), intent(in)   :: rec
+            integer(c_int64_t), value       :: nAtoms
+            real(c_double), value           :: precAngle
+            integer(c_int32_t), value       :: nPrec
+            type(d2bi_plan), intent(out)    :: plan
+        end function
+        integer(c_int) function d2bi_setup_tilt_sg(rec, k, sg, mxy, mzz, d) &
+                bind(c, name="d2bi_setup_tilt_sg")
+            import :: c_int, c_int32_t, c_double, d2bi_record
+            type(d2bi_record), intent(inout) :: rec
+            real(c_double), intent(in)       :: k(3)
+            real(c_double), value            :: sg
+            integer(c_int32_t), value        :: mxy, mzz
+            real(c_double), intent(in)       :: d(3)
+        end function
+        integer(c_int) function d2bi_setup_tilt_ur(rec, U, R, mxy, mzz, d) &
+                bind(c, name="d2bi_setup_tilt_ur")
+            import :: c_int, c_int32_t, c_double, d2bi_record
+            type(d2bi_record), intent(inout) :: rec
+            real(c_double), intent(in)       :: U(9), R(9)
+            integer(c_int32_t), value        :: mxy, mzz
+            real(c_double), intent(in)       :: d(3)
+        end function
+        integer(c_int) function d2bi_rotate_to_standard_frame(k, a_cell, hkl, U) &
+                bind(c, name="d2bi_rotate_to_standard_frame")
+            import :: c_int, c_double
+            real(c_double), intent(in)  :: k(3), a_cell(9), hkl(3)
+            real(c_double), intent(out) :: U(9)
+        end function
+        integer(c_int) function d2bi_gpu_atoms_upload(x, ld, nAtoms, atoms) &
+                bind(c, name="d2bi_gpu_atoms_upload")
+            import :: c_int, c_int64_t, c_double, c_ptr
+            real(c_double), intent(in)  :: x(*)
+            integer(c_int64_t), value   :: ld, nAtoms
+            type(c_ptr), intent(out)    :: atoms
+        end function
+        integer(c_int) function d2bi_gpu_atoms_free(atoms) bind(c, name="d2bi_gpu_atoms_free")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: atoms
+        end function
+        integer(c_int) function d2bi_gpu_image_dev(rec, atoms, precAngle, nPrec, mode, img, stats) &
+                bind(c, name="d2bi_gpu_image_dev")
+            import :: c_int, c_int32_t, c_double, c_ptr, d2bi_record, d2bi_stats
+            type(d2bi_record), intent(in)   :: rec
+            type(c_ptr), value              :: atoms
+            real(c_double), value           :: precAngle
+            integer(c_int32_t), value       :: nPrec, mode
+            real(c_double), intent(out)     :: img(*)
+            type(d2bi_stats), intent(out)   :: stats
+        end function
+        integer(c_int) function d2bi_gpu_blur(fin, nx, ny, t, fout) bind(c, name="d2bi_gpu_blur")
+            import :: c_int, c_int32_t, c_double
+            real(c_double), intent(in)  :: fin(*)
+            integer(c_int32_t), value   :: nx, ny
+            real(c_double), value       :: t
+            real(c_double), intent(out) :: fout(*)
+        end function
+        integer(c_int) function d2bi_write_png16(path, width, height, img) bind(c, name="d2bi_write_png16")
+            import :: c_int, c_char, c_float
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value              :: width, height
+            real(c_float), intent(in)          :: img(*)
+        end function
+        integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function
+    end interface
+
+contains
+
+    !   the library's last error message as a Fortran string
+    function d2bi_error_message() result(msg)
+        character(len=:), allocatable :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: s(:)
+        integer :: n, i
+        p = d2bi_last_error()
+        if (.not. c_associated(p)) then
+            msg = ""
+            return
+        end if
+        n = int(c_strlen(p))
+        call c_f_pointer(p, s, [n])
+        allocate(character(len=n) :: msg)
+        do i = 1, n
+            msg(i:i) = s(i)
+        end do
+    end function d2bi_error_message
+
+end module d2bi_gpu_mod

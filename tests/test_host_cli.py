@@ -1,0 +1,111 @@
+"""The Fortran d2bi host (simulated_tem_amd/host): command line, input and
+output files.  Error paths run on the CPU; imaging runs on the GPU."""
+import os
+import re
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "simulated_tem_amd", "host", "d2bi")
+
+needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="d2bi host not built (flang missing)")
+
+
+def run(args, cwd):
+    return subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def write_lammps(path, x, box):
+    with open(path, "w") as f:
+        f.write("LAMMPS data file written by the d2bi host tests\n\n%d atoms\n1 atom types\n\n" % len(x))
+        for i, ax in enumerate("xyz"):
+            f.write("%.10f %.10f %slo %shi\n" % (box[i][0], box[i][1], ax, ax))
+        f.write("\nMasses\n\n1 183.84\n\nAtoms # atomic\n\n")
+        for i, p in enumerate(x):
+            f.write("%d 1 %.16g %.16g %.16g 0 0 0\n" % (i + 1, p[0], p[1], p[2]))
+
+
+def write_xi(path, xi0, xig, hkl):
+    with open(path, "w") as f:
+        f.write("# extinction distances for the d2bi host tests\n")
+        f.write(" element          g        T (K)      V (V)         Re(U_g)         Im(U_g)            xi_g\n")
+        f.write("      W    0   0   0     300.000  200000.000     37.83095547      1.11167480 %15.8f\n" % xi0)
+        f.write("      W    1   1   0     300.000  200000.000     18.91115463      1.07133212    227.56411986\n")
+        f.write("      W  %3d %3d %3d     300.000  200000.000     13.94499223      1.03823539 %15.8f\n"
+                % (hkl[0], hkl[1], hkl[2], xig))
+
+
+@needs_exe
+def test_usage_and_errors(tmp_path):
+    r = run([], tmp_path)
+    assert r.returncode != 0 and "usage" in r.stdout
+    r = run(["-f", "missing.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1"], tmp_path)
+    assert r.returncode != 0 and "could not find input file" in (r.stdout + r.stderr)
+    (tmp_path / "a.lammps").write_text("x")
+    r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1", "-ng", "1"], tmp_path)
+    assert r.returncode != 0 and "not part of this build" in (r.stdout + r.stderr)
+    r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1", "-xifile", "x.dat",
+             "-xi0", "100"], tmp_path)
+    assert r.returncode != 0 and "xi_0 or xi_g" in (r.stdout + r.stderr)
+    r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0,1", "-k", "0,0,1"], tmp_path)
+    assert r.returncode != 0 and "g-vectors" in (r.stdout + r.stderr)
+
+
+def _read_png16(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w = 8, b"", None
+    while pos < len(data):
+        n = int.from_bytes(data[pos:pos + 4], "big")
+        typ = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        crc = int.from_bytes(data[pos + 8 + n:pos + 12 + n], "big")
+        assert zlib.crc32(typ + body) & 0xffffffff == crc
+        if typ == b"IHDR":
+            w, h = int.from_bytes(body[:4], "big"), int.from_bytes(body[4:8], "big")
+            assert body[8] == 16 and body[9] == 0
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = zlib.decompress(idat)
+    rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + 2 * w)
+    assert np.all(rows[:, 0] == 0)
+    px = rows[:, 1:].reshape(h, w, 2).astype(np.int64)
+    return px[:, :, 0] * 256 + px[:, :, 1]
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_golden_run_through_host(tmp_path, golden_dir):
+    """testD2BI.sh's run, with the golden image's rotation given by -U/-R
+    (the tilt search -ng is not in this build): minmaxavg within 1/256 of
+    0.000028/0.230127/0.081501, .dat within 1e-4 of the golden image."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    write_lammps(tmp_path / "w.lammps", w["x"], w["box"])
+    write_xi(tmp_path / "xi.dat", w["xi"][0], w["xi"][1], (2, 0, 0))
+    U = ",".join("%.6f" % v for v in w["rotation_cm"])
+    d = ",".join("%.3f" % v for v in w["space"])
+    for mode in ("fp64", "fast"):
+        r = run(["-f", "w.lammps", "-a0", "3.1652", "-lattice", "bcc", "-k", "0,0,1", "-g", "2,0,0",
+                 "-xifile", "xi.dat", "-U", U, "-R", "1,0,0,0,1,0,0,0,1", "-d", d, "-o", "out_" + mode,
+                 "-mode", mode], tmp_path)
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"minmaxavg d2bimg\s+(\S+)\s+(\S+)\s+(\S+)", r.stdout)
+        got = [float(v) for v in m.groups()]
+        for a, b in zip(got, (0.000028, 0.230127, 0.081501)):
+            assert abs(a - b) <= 0.0039
+        dats = [f for f in os.listdir(tmp_path) if f.startswith("out_" + mode) and f.endswith(".dat")]
+        assert len(dats) == 1
+        lines = open(tmp_path / dats[0]).read().splitlines()
+        assert lines[0].startswith("# d2bi v.")
+        nx, ny = (int(v) for v in lines[11].split())
+        vals = [float(ln[i:i + 12]) for ln in lines[12:] for i in range(0, len(ln), 12) if ln[i:i + 12].strip()]
+        img = np.array(vals).reshape(ny, nx)
+        assert img.shape == w["image"].shape
+        assert np.max(np.abs(img - w["image"])) < 1e-4
+        png = _read_png16(tmp_path / (dats[0][:-4] + ".png"))
+        assert png.shape == img.shape
+        assert png.min() == 0 and png.max() >= 65534     # normalised output spans the range
