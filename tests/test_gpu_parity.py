@@ -84,7 +84,15 @@ def test_field_fp64_membership_bitexact(case):
     assert np.max(np.abs(rho - orho)) < 1e-14
 
 
-def test_field_fast_membership_exact(case):
+@pytest.fixture(params=["0", "1"])
+def fast_variant(request, monkeypatch):
+    """Both FAST voxel kernels: one candidate list per 4x4x4 tile (0) and
+    per 2x2x2 group (1)."""
+    monkeypatch.setenv("D2BI_FAST_VARIANT", request.param)
+    return request.param
+
+
+def test_field_fast_membership_exact(case, fast_variant):
     name, x, im, oc = case
     ph, rho, cnt, r2m = im.field(x, "fast")
     oph, orho, ocnt, or2m = oc.field(nthreads=NTH)
@@ -92,20 +100,21 @@ def test_field_fast_membership_exact(case):
     occ = ocnt > 0
     assert np.max(np.abs(r2m[occ] - or2m[occ]) / np.maximum(or2m[occ], 1.0)) < 1e-5
     err = np.abs(ph - oph)
-    print("[fast field] %s max|dx| %.3e mean %.3e" % (name, err.max(), err.mean()), flush=True)
+    print("[fast field] %s v%s max|dx| %.3e mean %.3e" % (name, fast_variant, err.max(), err.mean()),
+          flush=True)
     assert np.max(np.abs(rho - orho)) < 1e-5
 
 
 @pytest.mark.parametrize("n_prec", [1, 5])
-def test_image_fp64_and_fast(case, n_prec):
+def test_image_fp64_and_fast(case, n_prec, fast_variant):
     name, x, im, oc = case
     ref = orc.Condition(im.rec, x, 0.005, n_prec).image(nthreads=NTH)
     g64 = im.computeD2BI_img(x, 0.005, n_prec, mode="fp64")
     gf = im.computeD2BI_img(x, 0.005, n_prec, mode="fast")
     e64 = np.max(np.abs(g64 - ref))
     ef = np.abs(gf - ref)
-    print("[image] %s P=%d fp64 max %.3e  fast max %.3e mean %.3e  I in [%.4f, %.4f]"
-          % (name, n_prec, e64, ef.max(), ef.mean(), ref.min(), ref.max()), flush=True)
+    print("[image] %s P=%d v%s fp64 max %.3e  fast max %.3e mean %.3e  I in [%.4f, %.4f]"
+          % (name, n_prec, fast_variant, e64, ef.max(), ef.mean(), ref.min(), ref.max()), flush=True)
     assert e64 <= FP64_TOL
     assert ef.max() <= FAST_TOL
 
@@ -190,13 +199,28 @@ def test_empty_and_single_atom():
     assert np.max(np.abs(img64 - ref)) <= FP64_TOL
 
 
-def test_batch_equals_single():
+@pytest.mark.parametrize("mode", ["fast", "fp64"])
+def test_batch_graph_replays_equal_eager(mode):
+    """d2bi_gpu_image_batch captures the whole batch into one HIP graph on the
+    first call and replays it afterwards.  Every replay, with eager work and
+    synchronisations in between, must equal the eager single-condition images
+    bit for bit (same kernels, deterministic sums)."""
     x, im1 = _cond("c0", cells=(10, 10, 10))
     _, im2 = _cond("c0", cells=(10, 10, 10), sg=0.015, density=True)
+    x3, im3 = _cond("c1", cells=(12, 9, 7))
     at = imaging.Atoms(x)
-    out = imaging.computeD2BI_img_batch([(im1, at, 0.005, 3), (im2, at, 0.005, 1)], mode="fast")
-    assert np.array_equal(out[0], im1.computeD2BI_img(at, 0.005, 3, mode="fast"))
-    assert np.array_equal(out[1], im2.computeD2BI_img(at, 0.005, 1, mode="fast"))
+    at3 = imaging.Atoms(x3)
+    conds = [(im1, at, 0.005, 3), (im2, at, 0.005, 1), (im3, at3, 0.004, 6)]
+    eager = [c[0].computeD2BI_img(c[1], c[2], c[3], mode=mode) for c in conds]
+    for rep in range(4):
+        out = imaging.computeD2BI_img_batch(conds, mode=mode)
+        for a, b in zip(out, eager):
+            assert np.array_equal(a, b), rep
+        # eager work between replays (a different condition through the same workspace)
+        im2.computeD2BI_img(at3 if rep % 2 else at, 0.005, 2, mode="fast")
+    ref = orc.Condition(im3.rec, x3, 0.004, 6).image()
+    tol = FP64_TOL if mode == "fp64" else FAST_TOL
+    assert np.max(np.abs(out[2] - ref)) <= tol
 
 
 def test_golden_w_loop_image(golden_dir):
