@@ -57,6 +57,52 @@ def cpu_baseline(im, x, sample_cols, nthreads):
                       (t_bin, w, w, p.Mz, t_win, nthreads, p.Mx, p.My, p.Mz)}
 
 
+def batch_bench(args, imaging, synth):
+    """Batched conditions through the graph-captured batch API: every
+    condition re-binned and re-voxelised, P precession points integrated per
+    voxel field.  Plain FP64 = eager per-condition calls on a subset."""
+    import numpy as np
+    cfg = dict(synth.CONFIGS[args.batch])
+    log("[bench] batch workload", args.batch)
+    x, box = synth.make_atoms(cfg)
+    x, conds = synth.conditions(cfg, x, box)
+    at = imaging.Atoms(x)
+    batch = [(im, at, pa, npr) for (im, pa, npr) in conds]
+    imaged = 0
+    for im, pa, npr in conds:
+        p = im.plan(len(x), pa, npr)
+        imaged += p.Mx * p.My * p.Mz * npr
+    t0 = time.perf_counter()
+    imgs = imaging.computeD2BI_img_batch(batch, mode="fast")          # plans, tunes, captures
+    log("[bench] batch first call (capture) %.2f s" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        imgs = imaging.computeD2BI_img_batch(batch, mode="fast")
+    t1 = time.perf_counter()
+    s_per_batch = (t1 - t0) / args.batch_steps
+    value = imaged / s_per_batch
+    log("[bench] batch fast: %.3f s/batch, %.3e imaged voxels/s" % (s_per_batch, value))
+    idx = np.linspace(0, len(conds) - 1, max(1, args.batch_fp64)).round().astype(int)
+    errs, imaged64 = [], 0
+    t64 = 0.0
+    for i in idx:
+        im, pa, npr = conds[i]
+        p = im.plan(len(x), pa, npr)
+        t0 = time.perf_counter()
+        g64 = im.computeD2BI_img(at, pa, npr, mode="fp64")
+        t64 += time.perf_counter() - t0
+        imaged64 += p.Mx * p.My * p.Mz * npr
+        errs.append(float(np.max(np.abs(g64 - imgs[i]))))
+    fp64_value = imaged64 / t64
+    log("[bench] batch fp64 (plain, %d conditions): %.3e imaged voxels/s, max err %.3e"
+        % (len(idx), fp64_value, max(errs)))
+    return {"workload": args.batch, "conditions": len(conds), "n_prec": conds[0][2],
+            "imaged_voxels_per_batch": imaged, "value": value, "unit": "imaged voxels/s",
+            "s_per_batch": s_per_batch, "steps": args.batch_steps,
+            "fp64_value": fp64_value, "fp64_conditions": len(idx),
+            "fast_over_fp64": value / fp64_value, "max_abs_err_vs_fp64": max(errs)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +112,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-cols", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", default="c3", help="batched workload ('' to skip)")
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-fp64", type=int, default=3, help="conditions timed in plain FP64")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build images on one GPU; multi-GPU runs are not part of it "
@@ -150,6 +199,8 @@ def main():
         "n_guard": int(st.n_guard),
         "roofline": roof,
     }
+    if args.batch:
+        out["batch"] = batch_bench(args, imaging, synth)
     if not args.no_cpu:
         nth = min(16, os.cpu_count() or 1)
         log("[bench] cpu baseline (%d threads)" % nth)

@@ -827,43 +827,45 @@ struct FastParams {
     VoxParams v;
     int ntx, nty, ntz;          // tiles per axis
     int ntiles, per_xcd;
-    float rc2_lo, rc2_hi;       // guard band around (3 sigma)^2
+    float e_rc, e_band;         // k2 (3 sigma)^2 and the guard half-width around it
     float rc_cull2;             // (3 sigma)^2 widened for the tile-box cull
     float k2;                   // -log2(e)/(2 sigma^2)
+    float w_scale;              // 2^e_rc
     float cellf[3];             // link-cell sides in FP32
 };
-
-__device__ inline float wave_min(float v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline float wave_max(float v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // One wave per 4x4x4 voxel tile.  The tile's candidates are the points of
 // every link cell in the unwrapped range [cell(first voxel)-1,
 // cell(last voxel)+1] on each axis; an unwrapped cell u stands for cell
 // mod(u, E) displaced by u - mod(u, E) cells, which is exactly the periodic
 // image neighbourList1b sees through its mod() (Lib_LinkCell3D.f90:1567-1575).
-// Candidates farther than 3 sigma from the tile's voxel box are dropped,
-// the rest are compacted into LDS (ballot + popcount) and every lane (voxel)
-// tests them all.  Position of a candidate relative to the tile's origin
-// cell: a*(frac + u - c0), computed in FP64 then rounded to FP32; of a voxel:
-// a*(zi + cell - c0).  Candidates two cells away from a voxel are always
-// beyond 3 sigma (link cells are >= 3 sigma wide); in the guard band the FP64
-// re-test also requires the neighbour offset to be -1, 0 or +1, as the
-// 27-cell loop does.  S is Kahan-compensated in FP32; cos/sin of g.r were
-// evaluated in FP64 during binning.
+// Cells, then points, farther than 3 sigma from the tile's voxel box are
+// dropped; the rest are compacted into LDS (ballot + popcount) and every lane
+// (voxel) tests them all.
+//
+// Staging.  The kept non-empty cells are listed once per tile with their
+// origin in the tile frame and the offset from candidate index to point
+// slot.  Candidates are then taken 64 at a time; the cell of candidate t is
+// the number of kept cells starting at or before t, less one, found from a
+// 64-bit mask of the cells starting inside the round (a flag per lane, a
+// ballot and a popcount).
+//
+// Arithmetic.  Positions are taken relative to the tile centre c (FP32,
+// rounded once from FP64 values).  For a candidate p and voxel q the Gaussian
+// exponent e = k2*|p - q|^2 (k2 = -log2(e)/(2 sigma^2), so the weight is 2^e)
+// is evaluated as
+//     e = (-2 k2 p).q + (k2 |p|^2 + k2 |q|^2)
+// from a per-candidate record (-2 k2 p, k2 |p|^2): three FMAs and an add.
+// Membership is e >= k2 (3 sigma)^2; where |e - k2 (3 sigma)^2| is within
+// the guard band the pair is re-tested in FP64 with the expression of
+// neighbourList1b (and the neighbour offset must be -1, 0 or +1, as in its
+// 27-cell loop), so the set of summed atoms is exactly the FP64 set.
+// S = sum w (cos, sin) is accumulated in blocks of 16 candidates whose sums
+// are merged with Kahan compensation.  cos/sin of g.r were evaluated in FP64.
 //
 // SUB = true splits the wave into eight 2x2x2 voxel groups; each group culls
-// the staged list again against its own voxel box (about 54 instead of 91
-// candidates per voxel for bcc at M = 4) and walks its own index list.
+// the staged list again against its own voxel box and walks its own index
+// list (about 54 instead of 91 candidates per voxel for bcc at M = 4).
 template <bool DENSITY, bool DIAG, bool SUB>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
                                                    const double4* __restrict__ frac,
@@ -896,122 +898,192 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const int c0x = P.ax.cell[0][x0], c0y = P.ax.cell[1][y0], c0z = P.ax.cell[2][z0];
     const int lcx = P.ax.cell[0][jx] - c0x, lcy = P.ax.cell[1][jy] - c0y, lcz = P.ax.cell[2][jz] - c0z;
     const double zix = P.ax.zi[0][jx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
-    const float qx = (float)(P.a[0] * (zix + lcx));
-    const float qy = (float)(P.a[1] * (ziy + lcy));
-    const float qz = (float)(P.a[2] * (ziz + lcz));
-    const float bx0 = wave_min(qx), bx1 = wave_max(qx);
-    const float by0 = wave_min(qy), by1 = wave_max(qy);
-    const float bz0 = wave_min(qz), bz1 = wave_max(qz);
-
-    // unwrapped cell range
-    const int ux0 = c0x - 1, uy0 = c0y - 1, uz0 = c0z - 1;
-    const int ncx = P.ax.cell[0][x1] + 1 - ux0 + 1;
-    const int ncy = P.ax.cell[1][y1] + 1 - uy0 + 1;
-    const int ncz = P.ax.cell[2][z1] + 1 - uz0 + 1;
-    const int ncell = ncx * ncy * ncz;
-
-    __shared__ int c_beg[TILE_MAXC];
-    __shared__ int c_off[TILE_MAXC + 1];
-    __shared__ float4 sp[TILE_CAP];            // x, y, z (A, rel. origin cell), cos
-    __shared__ float ssn[TILE_CAP];            // sin
-    __shared__ int sslot[TILE_CAP];            // slot
-    __shared__ int su[TILE_CAP];               // unwrapped cell offsets (dx + 8 dy + 64 dz)
-    __shared__ unsigned short sub[SUB ? 8 : 1][SUB ? TILE_CAP : 1];   // per-group candidate lists
-
-    // per-cell point ranges and their prefix sum (one cell per lane per round)
-    int carry = 0;
-    for (int cb = 0; cb < ncell; cb += 64) {
-        const int c = cb + lane;
-        int cnt = 0;
-        if (c < ncell) {
-            const int dx = c % ncx, dy = (c / ncx) % ncy, dz = c / (ncx * ncy);
-            const int rx = (ux0 + dx + P.Ex) % P.Ex, ry = (uy0 + dy + P.Ey) % P.Ey, rz = (uz0 + dz + P.Ez) % P.Ez;
-            const int rc = rx + P.Ex * (ry + P.Ey * rz);
-            const int b = cell_start[rc];
-            cnt = cell_start[rc + 1] - b;
-            c_beg[c] = b;
-        }
-        int incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
-        if (c < ncell) c_off[c] = carry + incl - cnt;
-        carry += __shfl(incl, 63, 64);
-    }
-    if (lane == 0) c_off[ncell] = carry;
-    __syncthreads();
-    const int T = carry;
-
+    // voxel position relative to the tile's origin cell (FP32)
+    auto qpos = [&](int a, int i, int c0) -> float {
+        return (float)(P.a[a] * (P.ax.zi[a][i] + (P.ax.cell[a][i] - c0)));
+    };
+    const float qx = qpos(0, jx, c0x), qy = qpos(1, jy, c0y), qz = qpos(2, jz, c0z);
+    // the tile's voxel box: positions do not decrease with the voxel index
+    // (link cells are monotonic along each axis), so its corners are the
+    // first and last voxels
+    const float bx0 = qpos(0, x0, c0x), bx1 = qpos(0, x1, c0x);
+    const float by0 = qpos(1, y0, c0y), by1 = qpos(1, y1, c0y);
+    const float bz0 = qpos(2, z0, c0z), bz1 = qpos(2, z1, c0z);
+    // tile-centred frame
+    const float ccx = 0.5f * (bx0 + bx1), ccy = 0.5f * (by0 + by1), ccz = 0.5f * (bz0 + bz1);
+    const float hx = bx1 - ccx, hy = by1 - ccy, hz = bz1 - ccz;
+    const float qcx = qx - ccx, qcy = qy - ccy, qcz = qz - ccz;
     const float k2 = F.k2;
+    const float kq2 = k2 * fmaf(qcx, qcx, fmaf(qcy, qcy, qcz * qcz));
+
+    // unwrapped cell range (cells are monotonic, so ux0 + dx lies in [-1, E])
+    const int ux0 = c0x - 1, uy0 = c0y - 1, uz0 = c0z - 1;
+    const int ncx = P.ax.cell[0][x1] - c0x + 3;
+    const int ncy = P.ax.cell[1][y1] - c0y + 3;
+    const int ncz = P.ax.cell[2][z1] - c0z + 3;
+    const int ncell = ncx * ncy * ncz;                 // <= TILE_MAXC (checked on the host)
+
+    __shared__ float4 crec[TILE_MAXC];         // kept cells: origin in the tile frame, slot - index
+    __shared__ int cupk[TILE_MAXC];            // kept cells: dx | dy << 8 | dz << 16
+    __shared__ int sflag[64];                  // cells starting at a lane's candidate this round
+    __shared__ float4 sA[2 * (TILE_CAP + 1)];  // per candidate: (-2 k2 p, k2 |p|^2) tile-centred,
+                                               // then (cos, sin) of g.r, slot, cell offsets; the
+                                               // last record is a sentinel that is never a member
+    __shared__ unsigned short sub[SUB ? 8 : 1][SUB ? TILE_CAP : 1];   // per-group candidate lists
+    int* const cstart = reinterpret_cast<int*>(sA);    // kept cells' first candidate (before staging)
+    static_assert(sizeof(int) * TILE_MAXC <= sizeof(float4) * 2 * TILE_CAP, "cstart overlay");
+
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long le = lt | (1ull << lane);
+
+    // kept cells: in range of the tile box and not empty, in cell order
+    int K = 0, T = 0;
+    {
+        const float inx = 1.0f / (float)ncx, inxy = 1.0f / (float)(ncx * ncy);
+        for (int cb = 0; cb < ncell; cb += 64) {
+            const int c = cb + lane;
+            int cnt = 0, b = 0, upk = 0;
+            float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+            if (c < ncell) {
+                // c = dx + ncx (dy + ncy dz); c < 128 so the float quotients are exact
+                const int dz = (int)(((float)c + 0.5f) * inxy);
+                const int r = c - dz * (ncx * ncy);
+                const int dy = (int)(((float)r + 0.5f) * inx);
+                const int dx = r - dy * ncx;
+                lx = F.cellf[0] * (float)(dx - 1) - ccx;
+                ly = F.cellf[1] * (float)(dy - 1) - ccy;
+                lz = F.cellf[2] * (float)(dz - 1) - ccz;
+                const float gx = fmaxf(0.0f, fmaxf(lx - hx, -hx - (lx + F.cellf[0])));
+                const float gy = fmaxf(0.0f, fmaxf(ly - hy, -hy - (ly + F.cellf[1])));
+                const float gz = fmaxf(0.0f, fmaxf(lz - hz, -hz - (lz + F.cellf[2])));
+                if (fmaf(gx, gx, fmaf(gy, gy, gz * gz)) <= F.rc_cull2) {
+                    int rx = ux0 + dx, ry = uy0 + dy, rz = uz0 + dz;
+                    rx = rx < 0 ? rx + P.Ex : (rx >= P.Ex ? rx - P.Ex : rx);
+                    ry = ry < 0 ? ry + P.Ey : (ry >= P.Ey ? ry - P.Ey : ry);
+                    rz = rz < 0 ? rz + P.Ez : (rz >= P.Ez ? rz - P.Ez : rz);
+                    const int rc = rx + P.Ex * (ry + P.Ey * rz);
+                    b = cell_start[rc];
+                    cnt = cell_start[rc + 1] - b;
+                    upk = dx | (dy << 8) | (dz << 16);
+                }
+            }
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += t;
+            }
+            const unsigned long long m = __ballot(cnt > 0);
+            if (cnt > 0) {
+                const int k = K + (int)__popcll(m & lt);
+                const int st = T + incl - cnt;
+                cstart[k] = st;
+                crec[k] = make_float4(lx, ly, lz, __int_as_float(b - st));
+                cupk[k] = upk;
+            }
+            K += (int)__popcll(m);
+            T += __shfl(incl, 63, 64);
+        }
+    }
+    if (lane == 0) {
+        sA[2 * TILE_CAP] = make_float4(0.0f, 0.0f, 0.0f, -__builtin_inff());
+        sA[2 * TILE_CAP + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
+    T = __builtin_amdgcn_readfirstlane(T);
+    K = __builtin_amdgcn_readfirstlane(K);
+    // starts of kept cells lane and lane + 64 (K <= 128)
+    const int s_lo = lane < K ? cstart[lane] : 0x7fffffff;
+    const int s_hi = lane + 64 < K ? cstart[lane + 64] : 0x7fffffff;
+    __syncthreads();
+
+    // exact FP64 test of neighbourList1b (Lib_LinkCell3D.f90:1565-1596) for
+    // a staged candidate: packed unwrapped cell offsets u, point slot
+    auto exact_member = [&](int u, int slot) -> bool {
+        const int kx = (u & 255) - 1 - lcx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
+        if (kx < -1 || kx > 1 || ky < -1 || ky > 1 || kz < -1 || kz > 1) return false;
+        const double4 f = frac[slot];
+        const double d1 = P.a[0] * (f.x + (kx - zix));
+        const double d2 = P.a[1] * (f.y + (ky - ziy));
+        const double d3 = P.a[2] * (f.z + (kz - ziz));
+        return ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
+    };
+    auto kahan_add = [](float& s, float& c, float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    };
     float sre = 0.0f, sim = 0.0f, cre = 0.0f, cim = 0.0f;
-    float r2m = 3.4028234663852886e38f;
+    float emax = -3.4028234663852886e38f;
     int cnt = 0;
     unsigned long long nguard = 0;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
-    int t_next = 0;
+    int t_next = 0, kbase = 0;
     while (t_next < T) {
         // stage: compact candidates within 3 sigma of the tile box
         int n = 0;
         while (t_next < T && n <= TILE_CAP - 64) {
+            // cell of candidate t_next + lane
+            sflag[lane] = 0;
+            __syncthreads();
+            if (s_lo >= t_next && s_lo < t_next + 64) sflag[s_lo - t_next] = 1;
+            if (s_hi >= t_next && s_hi < t_next + 64) sflag[s_hi - t_next] = 1;
+            __syncthreads();
+            const unsigned long long sm = __ballot(sflag[lane] != 0);
+            const int k = kbase + (int)__popcll(sm & le) - 1;
+            kbase += (int)__popcll(sm);
             const int t = t_next + lane;
             bool keep = false;
-            float4 pc;
-            float sn = 0.0f;
-            int slot = 0, upk = 0;
+            float4 rec, cs;
             if (t < T) {
-                int lo = 0, hi = ncell - 1;          // cell holding candidate t
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (c_off[mid] <= t) lo = mid; else hi = mid - 1;
-                }
-                const int c = lo;
-                slot = c_beg[c] + (t - c_off[c]);
-                const int dx = c % ncx, dy = (c / ncx) % ncy, dz = c / (ncx * ncy);
-                upk = dx + 8 * dy + 64 * dz;
-                pc = p32[slot];
-                pc.x += F.cellf[0] * (float)(dx - 1);
-                pc.y += F.cellf[1] * (float)(dy - 1);
-                pc.z += F.cellf[2] * (float)(dz - 1);
-                const float ex = fmaxf(0.0f, fmaxf(bx0 - pc.x, pc.x - bx1));
-                const float ey = fmaxf(0.0f, fmaxf(by0 - pc.y, pc.y - by1));
-                const float ez = fmaxf(0.0f, fmaxf(bz0 - pc.z, pc.z - bz1));
+                const float4 cr = crec[k];
+                const int slot = t + __float_as_int(cr.w);
+                const float4 pc = p32[slot];
+                const float px = pc.x + cr.x, py = pc.y + cr.y, pz = pc.z + cr.z;
+                const float ex = fmaxf(0.0f, fabsf(px) - hx);
+                const float ey = fmaxf(0.0f, fabsf(py) - hy);
+                const float ez = fmaxf(0.0f, fabsf(pz) - hz);
                 keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= F.rc_cull2;
-                if (keep) sn = sn32[slot];
+                if (keep) {
+                    const float m2k = -2.0f * k2;
+                    rec = make_float4(m2k * px, m2k * py, m2k * pz, k2 * fmaf(px, px, fmaf(py, py, pz * pz)));
+                    cs = make_float4(pc.w, sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]));
+                }
             }
             const unsigned long long m = __ballot(keep);
             if (keep) {
-                const int pos = n + (int)__popcll(m & lt);
-                sp[pos] = pc;
-                ssn[pos] = sn;
-                sslot[pos] = slot;
-                su[pos] = upk;
+                const int p = n + (int)__popcll(m & lt);
+                sA[2 * p] = rec;
+                sA[2 * p + 1] = cs;
             }
             n += (int)__popcll(m);
             t_next += 64;
         }
+        n = __builtin_amdgcn_readfirstlane(n);
         __syncthreads();
         int len = n;                            // candidates this lane walks
         if (SUB) {
             // group voxel box (xor-shuffles inside the 8-lane group)
-            float gx0 = qx, gx1 = qx, gy0 = qy, gy1 = qy, gz0 = qz, gz1 = qz;
+            float gx0 = qcx, gx1 = qcx, gy0 = qcy, gy1 = qcy, gz0 = qcz, gz1 = qcz;
 #pragma unroll
             for (int o = 1; o < 8; o <<= 1) {
                 gx0 = fminf(gx0, __shfl_xor(gx0, o, 64)); gx1 = fmaxf(gx1, __shfl_xor(gx1, o, 64));
                 gy0 = fminf(gy0, __shfl_xor(gy0, o, 64)); gy1 = fmaxf(gy1, __shfl_xor(gy1, o, 64));
                 gz0 = fminf(gz0, __shfl_xor(gz0, o, 64)); gz1 = fmaxf(gz1, __shfl_xor(gz1, o, 64));
             }
+            // (-2 k2 p) / (-2 k2) = p
+            const float inv = 1.0f / (-2.0f * k2);
             int gcnt = 0;
             for (int j0 = 0; j0 < n; j0 += 8) {
                 const int j = j0 + pos;
                 bool keep = false;
                 if (j < n) {
-                    const float4 pc = sp[j];
-                    const float ex = fmaxf(0.0f, fmaxf(gx0 - pc.x, pc.x - gx1));
-                    const float ey = fmaxf(0.0f, fmaxf(gy0 - pc.y, pc.y - gy1));
-                    const float ez = fmaxf(0.0f, fmaxf(gz0 - pc.z, pc.z - gz1));
+                    const float4 A = sA[2 * j];
+                    const float px = A.x * inv, py = A.y * inv, pz = A.z * inv;
+                    const float ex = fmaxf(0.0f, fmaxf(gx0 - px, px - gx1));
+                    const float ey = fmaxf(0.0f, fmaxf(gy0 - py, py - gy1));
+                    const float ez = fmaxf(0.0f, fmaxf(gz0 - pz, pz - gz1));
                     keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= F.rc_cull2;
                 }
                 const unsigned gb = (unsigned)(__ballot(keep) >> (8 * grp)) & 0xFFu;
@@ -1025,52 +1097,101 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
         if (SUB) {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o, 64));
+            lmax = __builtin_amdgcn_readfirstlane(lmax);
+            if (!DIAG) {
+                // pad every group list to the wave's longest with the sentinel
+                for (int k = len + pos; k < lmax; k += 8) sub[SUB ? grp : 0][k] = (unsigned short)TILE_CAP;
+                __syncthreads();
+            }
         }
-        // every voxel against every staged candidate (LDS broadcasts; per
-        // group lists when SUB)
-        for (int jj = 0; jj < lmax; jj++) {
-            const bool valid = jj < len;
-            const int j = SUB ? (valid ? (int)sub[SUB ? grp : 0][jj] : 0) : jj;
-            const float4 pc = sp[j];
-            const float dx = pc.x - qx, dy = pc.y - qy, dz = pc.z - qz;
-            const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            bool in = valid && act && dd <= F.rc2_lo;
-            const bool band = valid && act && dd > F.rc2_lo && dd <= F.rc2_hi;
-            if (__ballot(band)) {
-                if (band) {
-                    // exact FP64 test of neighbourList1b (Lib_LinkCell3D.f90:1565-1596)
-                    nguard++;
-                    const int u = su[j];
-                    const int kx = (u & 7) - 1 - lcx, ky = ((u >> 3) & 7) - 1 - lcy, kz = (u >> 6) - 1 - lcz;
-                    if (kx >= -1 && kx <= 1 && ky >= -1 && ky <= 1 && kz >= -1 && kz <= 1) {
-                        const double4 f = frac[sslot[j]];
-                        const double d1 = P.a[0] * (f.x + (kx - zix));
-                        const double d2 = P.a[1] * (f.y + (ky - ziy));
-                        const double d3 = P.a[2] * (f.z + (kz - ziz));
-                        in = ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
+        lmax = __builtin_amdgcn_readfirstlane(lmax);
+        if (DIAG) {
+            // every voxel against every staged candidate, per-pair exact
+            // membership with the counts and nearest ranges of the diagnostics
+            for (int jb = 0; jb < lmax; jb += 16) {
+                const int je = min(jb + 16, lmax);
+                float pr = 0.0f, pi = 0.0f;
+                for (int jj = jb; jj < je; jj++) {
+                    const bool valid = !SUB || jj < len;
+                    const int j = SUB ? (valid ? (int)sub[SUB ? grp : 0][jj] : 0) : jj;
+                    const float4 A = sA[2 * j];
+                    const float4 C = sA[2 * j + 1];
+                    const float e = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2)));
+                    bool in = valid && act && e >= F.e_rc;
+                    if (valid && act && fabsf(e - F.e_rc) <= F.e_band) {
+                        nguard++;
+                        in = exact_member(__float_as_int(C.w), __float_as_int(C.z));
+                    }
+                    const float w = in ? __builtin_amdgcn_exp2f(e) : 0.0f;
+                    pr = fmaf(w, C.x, pr);
+                    pi = fmaf(w, C.y, pi);
+                    emax = in ? fmaxf(emax, e) : emax;
+                    cnt += in ? 1 : 0;
+                }
+                kahan_add(sre, cre, pr);
+                kahan_add(sim, cim, pi);
+            }
+        } else {
+            // Branch-free pass over blocks of 16 candidates in the frame
+            // t = e - k2 (3 sigma)^2 (member iff t >= 0, weight 2^t scaled by
+            // 2^-e_rc), tracking the smallest |t|.  A block in which some
+            // lane came within the guard band is summed again with the exact
+            // FP64 test for those pairs (uniform branch, rare).
+            const float kq2r = kq2 - F.e_rc;
+            for (int jb = 0; jb < lmax; jb += 16) {
+                const int je = min(jb + 16, lmax);
+                float pr = 0.0f, pi = 0.0f, dmin = 3.4028234663852886e38f;
+#pragma unroll 4
+                for (int jj = jb; jj < je; jj++) {
+                    const int j = SUB ? (int)sub[SUB ? grp : 0][jj] : jj;
+                    const float4 A = sA[2 * j];
+                    const float4 C = sA[2 * j + 1];
+                    const float t = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2r)));
+                    const float w = t >= 0.0f ? __builtin_amdgcn_exp2f(t) : 0.0f;
+                    pr = fmaf(w, C.x, pr);
+                    pi = fmaf(w, C.y, pi);
+                    dmin = fminf(dmin, fabsf(t));
+                    if (DENSITY) emax = fmaxf(emax, t);
+                }
+                if (__ballot(dmin <= F.e_band)) {
+                    pr = 0.0f; pi = 0.0f;
+                    for (int jj = jb; jj < je; jj++) {
+                        const int j = SUB ? (int)sub[SUB ? grp : 0][jj] : jj;
+                        const float4 A = sA[2 * j];
+                        const float4 C = sA[2 * j + 1];
+                        const float t = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2r)));
+                        bool in = t >= 0.0f;
+                        if (fabsf(t) <= F.e_band) {
+                            nguard++;
+                            in = exact_member(__float_as_int(C.w), __float_as_int(C.z));
+                        }
+                        const float w = in ? __builtin_amdgcn_exp2f(t) : 0.0f;
+                        pr = fmaf(w, C.x, pr);
+                        pi = fmaf(w, C.y, pi);
                     }
                 }
+                kahan_add(sre, cre, pr);
+                kahan_add(sim, cim, pi);
             }
-            const float w = in ? __builtin_amdgcn_exp2f(dd * k2) : 0.0f;
-            // Kahan-compensated S += w (cos, sin)
-            const float yr = fmaf(w, pc.w, -cre);
-            const float tr = sre + yr;
-            cre = (tr - sre) - yr;
-            sre = tr;
-            const float yi = fmaf(w, ssn[j], -cim);
-            const float ti = sim + yi;
-            cim = (ti - sim) - yi;
-            sim = ti;
-            if (DENSITY || DIAG) r2m = in ? fminf(r2m, dd) : r2m;
-            if (DIAG) cnt += in ? 1 : 0;
         }
         __syncthreads();
     }
 
     if (act) {
         const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
+        if (!DIAG) {
+            // back from the t frame: weights were 2^t = 2^(e - e_rc)
+            sre *= F.w_scale;
+            sim *= F.w_scale;
+            emax += F.e_rc;
+        }
         const float mag = sqrtf(fmaf(sre, sre, sim * sim));
         const float ww = 1.0f / fmaxf(1.0e-16f, mag);
+        // nearest squared range from the largest exponent (huge(1.0) if none).
+        // Without DIAG the largest exponent is over every candidate, members
+        // or not: it differs from the members' only when no candidate is
+        // nearer than about 3 sigma, where the density is 0 either way.
+        const float r2m = emax > -3.4028234663852886e38f ? emax / k2 : 3.4028234663852886e38f;
         float rh = 1.0f;
         if (DENSITY && P.use_density) {
             // convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994) in FP32
@@ -1088,6 +1209,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             out_r2m[o] = r2m;
         }
     }
+    if (!act) nguard = 0;
     for (int o = 32; o > 0; o >>= 1) nguard += __shfl_down(nguard, o, 64);
     if (lane == 0 && nguard) atomicAdd(guard_count, nguard);
 }
@@ -1327,12 +1449,14 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     F.ntz = (p->Mz + TILE - 1) / TILE;
     F.ntiles = F.ntx * F.nty * F.ntz;
     F.per_xcd = (F.ntiles + 7) / 8;
-    // guard band: FP32 positions carry <= ~3e-6 A of rounding; 1e-5 relative
-    // on (3 sigma)^2 covers it with margin
-    F.rc2_lo = (float)(p->rc2 * (1.0 - 1.0e-5));
-    F.rc2_hi = (float)(p->rc2 * (1.0 + 1.0e-5));
+    // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
+    // rounded once from FP64, three FMAs); +-1e-5 relative covers it 5x
+    const double k2 = -p->i2s2 * 1.4426950408889634;
+    F.k2 = (float)k2;
+    F.e_rc = (float)(k2 * p->rc2);
+    F.e_band = (float)(std::fabs(k2 * p->rc2) * 1.0e-5);
+    F.w_scale = (float)std::exp2((double)F.e_rc);
     F.rc_cull2 = (float)(p->rc2 * (1.0 + 1.0e-4));
-    F.k2 = (float)(-p->i2s2 * 1.4426950408889634);
     F.cellf[0] = (float)p->a_col[0]; F.cellf[1] = (float)p->a_col[4]; F.cellf[2] = (float)p->a_col[8];
     // the unwrapped cell range of a tile must fit the per-tile cell table
     int worst = 1;
@@ -1393,7 +1517,10 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, TimeFn time_va
     const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
+    // each variant twice, the second (warm-cache) timing counts
     float t0 = time_variant(0), t1 = time_variant(1);
+    t0 = time_variant(0);
+    t1 = time_variant(1);
     if (t0 < 0 || t1 < 0) return seterr(D2BI_ERR_HIP, "variant timing failed");
     const int best = t1 < t0 ? 1 : 0;
     cache[key] = best;
@@ -1457,6 +1584,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         return ms;
     });
     if (variant < 0) return variant;
+    HIPCHK(hipMemsetAsync(gc, 0, 8, s));     // tuning launches counted too
     if (T) CHK(T->record(2));
     run_variant(variant);
     HIPCHK(hipGetLastError());

@@ -19,6 +19,10 @@ Workloads (sizes and flags):
       -> 640x640x1600 voxels
   c3  c1 atoms, nPrecAngle=32 (0.005 rad) x nTomoAngle=21 (20 deg)
       x g in {110, 011}: 42 voxel fields, 1344 integrations
+  c4  64 seeded configurations: sites log-uniform in [1e4, 4e6], box aspect
+      ratios U[0.5, 2], 0-128 centres, 0-16 voids, 1-4 distinct g from
+      {110, 1-10, 200, 211} (zone axis 001, or 01-1 for 211), s_g ~ U[0, 0.02], 1-8
+      precession points, M in {3, 4, 6}, image space = the periodic box
 """
 import ctypes as C
 import math
@@ -50,6 +54,47 @@ CONFIGS = {
 }
 CONFIGS["c3"] = dict(CONFIGS["c1"], g=[(1, 1, 0), (0, 1, 1)], n_prec=32, prec_angle=0.005,
                      n_tomo=21, tomo_angle=20.0)
+
+
+def c4_configs(seed=20240404, n=64, max_sites=4.0e6):
+    """The uneven batch: a list of workload dicts (each possibly with several
+    g-vectors)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    gset = [((1, 1, 0), (0, 0, 1)), ((1, -1, 0), (0, 0, 1)), ((2, 0, 0), (0, 0, 1)),
+            ((2, 1, 1), (0, 1, -1))]
+    for i in range(n):
+        sites = math.exp(rng.uniform(math.log(1.0e4), math.log(max_sites)))
+        ar = rng.uniform(0.5, 2.0, size=2)
+        cells3 = sites / 2.0
+        nz = max(3, int(round((cells3 / (ar[0] * ar[1])) ** (1.0 / 3.0))))
+        nx = max(3, int(round(ar[0] * nz)))
+        ny = max(3, int(round(ar[1] * nz)))
+        ng = int(rng.integers(1, 5))
+        picks = [gset[j] for j in rng.choice(4, size=ng, replace=False)]
+        a0 = 3.1652
+        box = (a0 * nx, a0 * ny, a0 * nz)
+        out.append(dict(a0=a0, cells=(nx, ny, nz), noise=0.05, n_centres=int(rng.integers(0, 129)),
+                        A=(1.0, 5.0), n_voids=int(rng.integers(0, 17)), r=(5.0, 20.0), zpad=False,
+                        gk=picks, sg=[float(rng.uniform(0.0, 0.02))], xi0=103.9, xig=207.5, V=200.0,
+                        M=int(rng.choice([3, 4, 6])), d=box, density=bool(rng.integers(0, 2)),
+                        n_prec=int(rng.integers(1, 9)), prec_angle=0.005, n_tomo=1, tomo_angle=20.0,
+                        seed=seed * 1000 + i))
+    return out
+
+
+def c4_conditions(cfgs):
+    """Per configuration: (x, [(imaging, precAngle, nPrec), ...])."""
+    out = []
+    for cfg in cfgs:
+        x, box = make_atoms(cfg)
+        conds = []
+        for g, k in cfg["gk"]:
+            c = dict(cfg, g=[g], k=k)
+            x2, cc = conditions(c, x, box)
+            conds.extend(cc)
+        out.append((x2, conds))
+    return out
 
 
 def scaled(name, cells=None, d=None):

@@ -223,6 +223,29 @@ def test_batch_graph_replays_equal_eager(mode):
     assert np.max(np.abs(out[2] - ref)) <= tol
 
 
+def test_uneven_batch_c4_style():
+    """A small uneven batch like c4: mixed M (3, 4, 6), zone axes 001 and
+    01-1, g in {110, 1-10, 200, 211}, 1-8 precession points, density on and
+    off, all in one graph; each image against the oracle."""
+    cfgs = [c for c in synth.c4_configs(seed=77, n=40, max_sites=3.0e4)][:6]
+    data = synth.c4_conditions(cfgs)
+    batch, refs = [], []
+    for x, conds in data:
+        at = imaging.Atoms(x)
+        for im, pa, npr in conds:
+            batch.append((im, at, pa, npr))
+            refs.append(orc.Condition(im.rec, x, pa, npr).image(nthreads=NTH))
+    assert len(set(c["M"] for c in cfgs)) >= 2
+    out = imaging.computeD2BI_img_batch(batch, mode="fast")
+    out2 = imaging.computeD2BI_img_batch(batch, mode="fast")          # graph replay
+    for a, b, r in zip(out, out2, refs):
+        assert np.array_equal(a, b)
+        assert np.max(np.abs(a - r)) <= FAST_TOL
+    out64 = imaging.computeD2BI_img_batch(batch, mode="fp64")
+    for a, r in zip(out64, refs):
+        assert np.max(np.abs(a - r)) <= FP64_TOL
+
+
 def test_golden_w_loop_image(golden_dir):
     """Data/test/...ng1.00.dat via -U <header RU> -R I -d <header extents>: the
     header holds RU to 6 decimals and the extents to 3, so the reproduction

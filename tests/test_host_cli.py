@@ -101,8 +101,11 @@ def test_golden_run_through_host(tmp_path, golden_dir):
         assert len(dats) == 1
         lines = open(tmp_path / dats[0]).read().splitlines()
         assert lines[0].startswith("# d2bi v.")
-        nx, ny = (int(v) for v in lines[11].split())
-        vals = [float(ln[i:i + 12]) for ln in lines[12:] for i in range(0, len(ln), 12) if ln[i:i + 12].strip()]
+        h = next(i for i, ln in enumerate(lines) if not ln.startswith("#"))
+        assert any(ln.startswith("# rotation") for ln in lines[:h])
+        nx, ny = (int(v) for v in lines[h].split())
+        vals = [float(ln[i:i + 12]) for ln in lines[h + 1:] for i in range(0, len(ln), 12)
+                if ln[i:i + 12].strip()]
         img = np.array(vals).reshape(ny, nx)
         assert img.shape == w["image"].shape
         assert np.max(np.abs(img - w["image"])) < 1e-4
