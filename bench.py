@@ -10,7 +10,7 @@ Mx*My*Mz*nPrecessionAngle.  The timed steps run the FAST (FP32) mode; the
 plain per-condition FP64 mode is timed beside it (fp64_value) with the FAST
 vs FP64 intensity error of the same image.
 
-Prints ONE JSON line.  roofline: the dominant kernel (k_voxel_fast), timed
+Prints ONE JSON line.  roofline: the dominant kernel (k_voxel_tile), timed
 live with HIP events around its launch on the library's stream; algorithmic
 work per voxel = 388 FP32 flops + 30 transcendentals (28.27 in-sphere atoms
 x 13 flops + 1 exp, plus normalisation and density).  cpu_baseline: the
@@ -103,6 +103,23 @@ def batch_bench(args, imaging, synth):
             "fast_over_fp64": value / fp64_value, "max_abs_err_vs_fp64": max(errs)}
 
 
+def calibrate_copy(mib, reps=3):
+    """A plain device-to-device copy of known size, so that a profiler's
+    FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "reps": reps,
+                      "GB_per_s": 8 * n / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,12 +132,18 @@ def main():
     ap.add_argument("--batch", default="c3", help="batched workload ('' to skip)")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-fp64", type=int, default=3, help="conditions timed in plain FP64")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build images on one GPU; multi-GPU runs are not part of it "
               "(conditions or image tiles would shard across GPUs with no reduction).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import numpy as np
     from simulated_tem_amd import imaging, synth
@@ -176,7 +199,7 @@ def main():
 
     t_vox = float(np.mean(ms_vox)) * 1e-3
     achieved = vox * FLOPS_PER_VOXEL / t_vox / 1e12
-    roof = {"bound": "mfma", "pipe": "valu_fp32", "kernel": "k_voxel_fast",
+    roof = {"bound": "mfma", "pipe": "valu_fp32", "kernel": "k_voxel_tile",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
             "ms_per_launch": t_vox * 1e3,

@@ -136,6 +136,22 @@ int d2bi_setup_tilt_sg(d2bi_record* rec, const double k[3], double sg_target,
 int d2bi_setup_tilt_ur(d2bi_record* rec, const double U[9], const double R[9],
                        int32_t mxy, int32_t mzz, const double d[3]);
 
+/* setSupercellTilt1a without atom positions
+ * (Lib_DynamicalTwoBeamImaging.f90:943-1045, with setSupercellTilt1 :795-939,
+ * findBestRotationMatrix and tiltToDarkField,
+ * Lib_DiffractionConditions.f90:129-357, 751-927): the (g, ng) two-beam
+ * condition.  The crystal rotation U and tilt R put the reflection ng*g at
+ * the Bragg condition with the weakest summed two-beam intensity of all other
+ * permitted reflections within |hkl| <= 10, scored with the extinction
+ * distances of the table xifile, then tilt about y to the dark-field minimum;
+ * a fractional ng interpolates floor/ceiling by quaternion slerp.  On entry
+ * rec holds a0, a_cell, g, E, xi0, xig (for g), super_a/super_n; lattice is
+ * 1 sc, 2 bcc, 3 fcc, 4 hcp; d is the voxel cuboid.  On return U, R, delta,
+ * uvw, gk_a, gk_n and sg are set and d[2] is the depth after the tilt
+ * (voxelSupercellDimensions(3)).  ng >= 1. */
+int d2bi_setup_tilt_ng(d2bi_record* rec, const double k[3], double ng, int32_t lattice,
+                       const char* xifile, int32_t mxy, int32_t mzz, double d[3]);
+
 /* rotateToStandardFrame (Lib_DiffractionConditions.f90:65-108). */
 int d2bi_rotate_to_standard_frame(const double k[3], const double a_cell[9],
                                   const double hkl[3], double U[9]);

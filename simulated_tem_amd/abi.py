@@ -90,7 +90,7 @@ class D2biSynthParams(C.Structure):
 # Every symbol include/d2bi_gpu.h declares, for the export check.
 EXPORTED = [
     "d2bi_gpu_init", "d2bi_gpu_finalize", "d2bi_last_error", "d2bi_version",
-    "d2bi_plan_image", "d2bi_setup_tilt_sg", "d2bi_setup_tilt_ur",
+    "d2bi_plan_image", "d2bi_setup_tilt_sg", "d2bi_setup_tilt_ur", "d2bi_setup_tilt_ng",
     "d2bi_rotate_to_standard_frame", "d2bi_wavelength", "d2bi_velocity",
     "d2bi_deviation_parameter",
     "d2bi_gpu_atoms_upload", "d2bi_gpu_atoms_free", "d2bi_gpu_atoms_count",
@@ -124,6 +124,8 @@ def load():
     L.d2bi_setup_tilt_sg.argtypes = [C.POINTER(D2biRecord), _dp, C.c_double, C.c_int32,
                                      C.c_int32, _dp]
     L.d2bi_setup_tilt_ur.argtypes = [C.POINTER(D2biRecord), _dp, _dp, C.c_int32, C.c_int32, _dp]
+    L.d2bi_setup_tilt_ng.argtypes = [C.POINTER(D2biRecord), _dp, C.c_double, C.c_int32, C.c_char_p,
+                                     C.c_int32, C.c_int32, _dp]
     L.d2bi_rotate_to_standard_frame.argtypes = [_dp, _dp, _dp, _dp]
     L.d2bi_wavelength.restype = C.c_double
     L.d2bi_wavelength.argtypes = [C.c_double]

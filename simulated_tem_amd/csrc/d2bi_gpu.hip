@@ -99,7 +99,7 @@ struct Workspace {
     Buf scan_tmp;
     Buf counters;               // misc device counters (guard count, max per cell)
     // voxel axis tables
-    Buf ax_cell, ax_zi, ax_start;
+    Buf ax_cell, ax_zi, ax_start, ax_ql;
     // fields and image
     Buf f64_re, f64_im, f64_rho, f32_field;
     Buf cnt, r2m;
@@ -112,7 +112,7 @@ struct Workspace {
     {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
-                      &slot_p32, &slot_sn32, &scan_tmp, &counters, &ax_cell, &ax_zi, &ax_start, &f64_re,
+                      &slot_p32, &slot_sn32, &scan_tmp, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_k};
         for (Buf* b : all) b->release();
@@ -289,6 +289,14 @@ struct BinParams {
     int64_t n;                  // atoms
     int generic;                // 1: LinkCell3D add1 (no transform, no filter)
     double cell_a[3];           // diagonal of the link-cell matrix (FAST-mode records)
+    // replica pre-classification: c ~= M x + off[nn] approximates the
+    // link-cell coordinates of replica nn (products folded, so a few
+    // roundings away from the reference's order); a replica is decided
+    // from it unless a coordinate lies within eps of a box face
+    int prefilter;
+    double M[9];                // ia . RU (column-major)
+    double off[3 * MAX_REPL];   // ia . (RU . Ruvw[nn] + delta - lc_delta)
+    double mabs, oabs;          // magnitude bounds: per unit |x|, and the rest
 };
 
 // Link-cell index of a reduced coordinate: mod(floor(y) + 16*N, N)
@@ -349,9 +357,31 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __rest
         uint32_t m = 0;
         if (j < P.n) {
             double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-            for (int nn = 0; nn < P.nuvw; nn++) {
-                double yy[3], cyy[3];
-                if (bin_point(P, x3, nn, yy, cyy)) m |= 1u << nn;
+            if (P.prefilter) {
+                double cb[3];
+                for (int i = 0; i < 3; i++) cb[i] = P.M[i] * x3[0] + P.M[3 + i] * x3[1] + P.M[6 + i] * x3[2];
+                // |approximate - exact| is a few units of roundoff (1e-16) of
+                // the magnitudes involved; eps is 1e-10 of their bound
+                const double eps =
+                    1.0e-10 * (1.0 + P.mabs * (fabs(x3[0]) + fabs(x3[1]) + fabs(x3[2])) + P.oabs);
+                const double E[3] = {(double)P.Ex, (double)P.Ey, (double)P.Ez};
+                for (int nn = 0; nn < P.nuvw; nn++) {
+                    bool out = false, sure = true;
+                    for (int i = 0; i < 3; i++) {
+                        const double c = cb[i] + P.off[3 * nn + i];
+                        out = out || c <= -eps || c >= E[i] + eps;
+                        sure = sure && (c <= -eps || c >= E[i] + eps || (c >= eps && c <= E[i] - eps));
+                    }
+                    if (out) continue;                       // outside on some axis for certain
+                    if (sure) { m |= 1u << nn; continue; }   // inside on every axis for certain
+                    double yy[3], cyy[3];
+                    if (bin_point(P, x3, nn, yy, cyy)) m |= 1u << nn;
+                }
+            } else {
+                for (int nn = 0; nn < P.nuvw; nn++) {
+                    double yy[3], cyy[3];
+                    if (bin_point(P, x3, nn, yy, cyy)) m |= 1u << nn;
+                }
             }
             mask[j] = m;
         }
@@ -564,6 +594,45 @@ static int bin_params(const d2bi_record* r, const d2bi_plan* p, BinParams* P)
     P->Ex = p->Ex; P->Ey = p->Ey; P->Ez = p->Ez;
     P->generic = 0;
     P->cell_a[0] = p->a_col[0]; P->cell_a[1] = p->a_col[4]; P->cell_a[2] = p->a_col[8];
+    // folded transform of the pre-classification (any rounding order: it
+    // only decides replicas that are further than eps from a box face)
+    double ia_max = 0.0, ru_max = 0.0, t_max = 0.0;
+    for (int i = 0; i < 9; i++) {
+        ia_max = std::max(ia_max, std::fabs(P->ia[i]));
+        ru_max = std::max(ru_max, std::fabs(P->RU[i]));
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double v = 0.0;
+            for (int k = 0; k < 3; k++) v += P->ia[3 * k + i] * P->RU[3 * j + k];
+            P->M[3 * j + i] = v;
+        }
+    for (int nn = 0; nn < P->nuvw; nn++) {
+        double z[3];
+        for (int k = 0; k < 3; k++) {
+            double v = P->delta[k] - P->lc_delta[k];
+            double t = std::fabs(P->delta[k]) + std::fabs(P->lc_delta[k]);
+            for (int j = 0; j < 3; j++) {
+                v += P->RU[3 * j + k] * P->Ruvw[3 * nn + j];
+                t += std::fabs(P->RU[3 * j + k] * P->Ruvw[3 * nn + j]) + std::fabs(P->Ruvw[3 * nn + j]);
+            }
+            z[k] = v;
+            t_max = std::max(t_max, t);
+        }
+        for (int i = 0; i < 3; i++) {
+            double v = 0.0;
+            for (int k = 0; k < 3; k++) v += P->ia[3 * k + i] * z[k];
+            P->off[3 * nn + i] = v;
+        }
+    }
+    // bounds on the magnitude of every intermediate of both evaluations:
+    // |x| terms scale with 3 max|ia| max(1, max|RU|), the rest with
+    // 3 max|ia| (|RU Ruvw| + |Ruvw| + |delta| + |lc_delta|)
+    P->mabs = 3.0 * ia_max * std::max(1.0, ru_max) * std::max(1.0, 3.0 * ru_max);
+    P->oabs = 3.0 * ia_max * t_max;
+    // box faces at 0 and E <= 2^20, so that E + eps and the exact path's
+    // integer test are exact
+    P->prefilter = (P->Ex <= (1 << 20) && P->Ey <= (1 << 20) && P->Ez <= (1 << 20)) ? 1 : 0;
     return D2BI_OK;
 }
 
@@ -576,9 +645,19 @@ static int bin_params(const d2bi_record* r, const d2bi_plan* p, BinParams* P)
 //   cell = mod(floor(zi) + 16*Ex, Ex), zi = zi - cell
 // (neighbourList1b, Lib_LinkCell3D.f90:1549-1565).  The zero off-diagonal
 // terms of the reference's 3x3 products add +-0 and change no bit.
+static constexpr int TILE = 4;                       // FAST voxel tiles: 4x4x4 voxels = one wave64
+
+// FP32 voxel position along one axis relative to the link cell of its
+// FAST tile's first voxel, and that cell offset
+struct QL {
+    float q;
+    int lc;
+};
+
 struct AxisTables {
     std::vector<int> cell[3];
     std::vector<double> zi[3];
+    std::vector<QL> ql[3];
     std::vector<int> start[3];     // first voxel of each link cell (size E+1)
     int lo[3], hi[3];              // active link-cell range
 };
@@ -609,6 +688,12 @@ static int axis_tables(const d2bi_record* r, const d2bi_plan* p, AxisTables* t)
         for (int i = 1; i < M[a]; i++)
             if (t->cell[a][i] < t->cell[a][i - 1])
                 return seterr(D2BI_ERR_ARG, "voxel column wraps the link-cell box (axis %d)", a);
+        t->ql[a].resize(M[a]);
+        const double side = p->a_col[4 * a];
+        for (int i = 0; i < M[a]; i++) {
+            const int lc = t->cell[a][i] - t->cell[a][(i / TILE) * TILE];
+            t->ql[a][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
+        }
         for (int i = M[a] - 1; i >= 0; i--) t->start[a][t->cell[a][i]] = i;
         // cells with no voxels take the start of the next occupied cell
         for (int c = E[a] - 1; c >= 0; c--)
@@ -624,6 +709,7 @@ struct AxisDev {
     const int* cell[3];
     const double* zi[3];
     const int* start[3];
+    const QL* ql[3];
 };
 
 static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
@@ -634,18 +720,22 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
     CHK(w.ax_cell.ensure(sizeof(int) * nc));
     CHK(w.ax_zi.ensure(sizeof(double) * nc));
     CHK(w.ax_start.ensure(sizeof(int) * ns));
-    size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns;
+    CHK(w.ax_ql.ensure(sizeof(QL) * nc));
+    size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns + sizeof(QL) * nc;
     void* hp;
     CHK(pinned(bytes, &hp));
     char* h = (char*)hp;
     int* hc = (int*)h;
     double* hz = (double*)(h + sizeof(int) * nc);
     int* hs = (int*)(h + sizeof(int) * nc + sizeof(double) * nc);
+    QL* hq = (QL*)(h + sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns);
     size_t oc = 0, os = 0;
     for (int a = 0; a < 3; a++) {
         std::memcpy(hc + oc, t.cell[a].data(), sizeof(int) * t.cell[a].size());
         std::memcpy(hz + oc, t.zi[a].data(), sizeof(double) * t.zi[a].size());
         std::memcpy(hs + os, t.start[a].data(), sizeof(int) * t.start[a].size());
+        std::memcpy(hq + oc, t.ql[a].data(), sizeof(QL) * t.ql[a].size());
+        d->ql[a] = w.ax_ql.as<QL>() + oc;
         d->cell[a] = w.ax_cell.as<int>() + oc;
         d->zi[a] = w.ax_zi.as<double>() + oc;
         d->start[a] = w.ax_start.as<int>() + os;
@@ -655,6 +745,7 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
     HIPCHK(hipMemcpyAsync(w.ax_cell.p, hc, sizeof(int) * nc, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.ax_zi.p, hz, sizeof(double) * nc, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.ax_start.p, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.ax_ql.p, hq, sizeof(QL) * nc, hipMemcpyHostToDevice, s));
     // the pinned buffer is reused by later calls: wait until the copies are done
     HIPCHK(hipStreamSynchronize(s));
     return D2BI_OK;
@@ -819,7 +910,6 @@ __global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const doubl
 
 // ------------------------------------------------------------------ (b) voxelise, FAST
 
-static constexpr int TILE = 4;                       // 4x4x4 voxels = one wave64
 static constexpr int TILE_CAP = 160;                 // staged candidates per pass (LDS ~5 KB/wave)
 static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
 
@@ -831,6 +921,8 @@ struct FastParams {
     float rc_cull2;             // (3 sigma)^2 widened for the tile-box cull
     float k2;                   // -log2(e)/(2 sigma^2)
     float w_scale;              // 2^e_rc
+    float m2_min;               // (1e-16)^2 / w_scale^2
+    float inv_k2;               // 1 / k2
     float cellf[3];             // link-cell sides in FP32
 };
 
@@ -894,21 +986,19 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const bool act = ix <= x1 && iy <= y1 && iz <= z1;
     const int jx = min(ix, x1), jy = min(iy, y1), jz = min(iz, z1);
 
-    // origin cell of the tile and the voxel's own cell / offset
+    // origin cell of the tile; the voxel's FP32 position relative to it and
+    // its cell offset (host tables)
     const int c0x = P.ax.cell[0][x0], c0y = P.ax.cell[1][y0], c0z = P.ax.cell[2][z0];
-    const int lcx = P.ax.cell[0][jx] - c0x, lcy = P.ax.cell[1][jy] - c0y, lcz = P.ax.cell[2][jz] - c0z;
-    const double zix = P.ax.zi[0][jx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
-    // voxel position relative to the tile's origin cell (FP32)
-    auto qpos = [&](int a, int i, int c0) -> float {
-        return (float)(P.a[a] * (P.ax.zi[a][i] + (P.ax.cell[a][i] - c0)));
-    };
-    const float qx = qpos(0, jx, c0x), qy = qpos(1, jy, c0y), qz = qpos(2, jz, c0z);
+    const QL qlx = P.ax.ql[0][jx], qly = P.ax.ql[1][jy], qlz = P.ax.ql[2][jz];
+    const int lcx = qlx.lc, lcy = qly.lc, lcz = qlz.lc;
+    const float qx = qlx.q, qy = qly.q, qz = qlz.q;
     // the tile's voxel box: positions do not decrease with the voxel index
     // (link cells are monotonic along each axis), so its corners are the
     // first and last voxels
-    const float bx0 = qpos(0, x0, c0x), bx1 = qpos(0, x1, c0x);
-    const float by0 = qpos(1, y0, c0y), by1 = qpos(1, y1, c0y);
-    const float bz0 = qpos(2, z0, c0z), bz1 = qpos(2, z1, c0z);
+    const QL ex0 = P.ax.ql[0][x0], ex1 = P.ax.ql[0][x1];
+    const QL ey0 = P.ax.ql[1][y0], ey1 = P.ax.ql[1][y1];
+    const QL ez0 = P.ax.ql[2][z0], ez1 = P.ax.ql[2][z1];
+    const float bx0 = ex0.q, bx1 = ex1.q, by0 = ey0.q, by1 = ey1.q, bz0 = ez0.q, bz1 = ez1.q;
     // tile-centred frame
     const float ccx = 0.5f * (bx0 + bx1), ccy = 0.5f * (by0 + by1), ccz = 0.5f * (bz0 + bz1);
     const float hx = bx1 - ccx, hy = by1 - ccy, hz = bz1 - ccz;
@@ -918,9 +1008,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
 
     // unwrapped cell range (cells are monotonic, so ux0 + dx lies in [-1, E])
     const int ux0 = c0x - 1, uy0 = c0y - 1, uz0 = c0z - 1;
-    const int ncx = P.ax.cell[0][x1] - c0x + 3;
-    const int ncy = P.ax.cell[1][y1] - c0y + 3;
-    const int ncz = P.ax.cell[2][z1] - c0z + 3;
+    const int ncx = ex1.lc + 3, ncy = ey1.lc + 3, ncz = ez1.lc + 3;
     const int ncell = ncx * ncy * ncz;                 // <= TILE_MAXC (checked on the host)
 
     __shared__ float4 crec[TILE_MAXC];         // kept cells: origin in the tile frame, slot - index
@@ -1003,6 +1091,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
         const int kx = (u & 255) - 1 - lcx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
         if (kx < -1 || kx > 1 || ky < -1 || ky > 1 || kz < -1 || kz > 1) return false;
         const double4 f = frac[slot];
+        const double zix = P.ax.zi[0][jx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
         const double d1 = P.a[0] * (f.x + (kx - zix));
         const double d2 = P.a[1] * (f.y + (ky - ziy));
         const double d3 = P.a[2] * (f.z + (kz - ziz));
@@ -1017,7 +1106,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     float sre = 0.0f, sim = 0.0f, cre = 0.0f, cim = 0.0f;
     float emax = -3.4028234663852886e38f;
     int cnt = 0;
-    unsigned long long nguard = 0;
+    unsigned nguard = 0;
 
     int t_next = 0, kbase = 0;
     while (t_next < T) {
@@ -1132,14 +1221,14 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 kahan_add(sim, cim, pi);
             }
         } else {
-            // Branch-free pass over blocks of 16 candidates in the frame
+            // Branch-free pass over blocks of 32 candidates in the frame
             // t = e - k2 (3 sigma)^2 (member iff t >= 0, weight 2^t scaled by
             // 2^-e_rc), tracking the smallest |t|.  A block in which some
             // lane came within the guard band is summed again with the exact
             // FP64 test for those pairs (uniform branch, rare).
             const float kq2r = kq2 - F.e_rc;
-            for (int jb = 0; jb < lmax; jb += 16) {
-                const int je = min(jb + 16, lmax);
+            for (int jb = 0; jb < lmax; jb += 32) {
+                const int je = min(jb + 32, lmax);
                 float pr = 0.0f, pi = 0.0f, dmin = 3.4028234663852886e38f;
 #pragma unroll 4
                 for (int jj = jb; jj < je; jj++) {
@@ -1179,19 +1268,19 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
 
     if (act) {
         const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
-        if (!DIAG) {
-            // back from the t frame: weights were 2^t = 2^(e - e_rc)
-            sre *= F.w_scale;
-            sim *= F.w_scale;
-            emax += F.e_rc;
-        }
-        const float mag = sqrtf(fmaf(sre, sre, sim * sim));
-        const float ww = 1.0f / fmaxf(1.0e-16f, mag);
+        // x = S / max(|S|, 1e-16).  Without DIAG the sums carry the factor
+        // 2^-e_rc of the t frame (weights 2^t = 2^(e - e_rc)); it cancels in
+        // S/|S| and moves the threshold.
+        const float m2 = fmaf(sre, sre, sim * sim);
+        const float ww = m2 > (DIAG ? 1.0e-32f : F.m2_min) ? __builtin_amdgcn_rsqf(m2)
+                                                          : (DIAG ? 1.0e16f : 1.0e16f * F.w_scale);
         // nearest squared range from the largest exponent (huge(1.0) if none).
         // Without DIAG the largest exponent is over every candidate, members
         // or not: it differs from the members' only when no candidate is
         // nearer than about 3 sigma, where the density is 0 either way.
-        const float r2m = emax > -3.4028234663852886e38f ? emax / k2 : 3.4028234663852886e38f;
+        float r2m;
+        if (DIAG) r2m = emax > -3.4028234663852886e38f ? emax / k2 : 3.4028234663852886e38f;
+        else r2m = emax > -3.4028234663852886e38f ? (emax + F.e_rc) * F.inv_k2 : 3.4028234663852886e38f;
         float rh = 1.0f;
         if (DENSITY && P.use_density) {
             // convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994) in FP32
@@ -1203,15 +1292,14 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
             }
         }
-        field[o] = make_float4(sre * ww, sim * ww, rh, 0.0f);
+        if (DENSITY) field[o] = make_float4(sre * ww, sim * ww, rh, 0.0f);
+        else reinterpret_cast<float2*>(field)[o] = make_float2(sre * ww, sim * ww);
         if (DIAG) {
             out_cnt[o] = cnt;
             out_r2m[o] = r2m;
         }
     }
-    if (!act) nguard = 0;
-    for (int o = 32; o > 0; o >>= 1) nguard += __shfl_down(nguard, o, 64);
-    if (lane == 0 && nguard) atomicAdd(guard_count, nguard);
+    if (act && nguard) atomicAdd(guard_count, (unsigned long long)nguard);
 }
 
 // ------------------------------------------------------------------ (c) integrate, FAST
@@ -1227,8 +1315,8 @@ struct FastIntParams {
 // precession changes s_g only).  FP32 classic RK4 with h = 2 dz; the
 // per-step renormalisation of the reference is a positive scalar on a
 // linear map, so it is applied once at the end: I = |phi_g|^2/|phi|^2.
-template <int NP>
-__global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const float4* __restrict__ field,
+template <int NP, bool DENS>
+__global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const void* __restrict__ field,
                                                         double* __restrict__ img, int accumulate,
                                                         double inv_total)
 {
@@ -1240,12 +1328,21 @@ __global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const f
     float p0r[NP], p0i[NP], pgr[NP], pgi[NP];
 #pragma unroll
     for (int p = 0; p < NP; p++) { p0r[p] = 1.0f; p0i[p] = 0.0f; pgr[p] = 0.0f; pgi[p] = 0.0f; }
+    // field: float4 {x, rho} with the density field, else float2 x (rho = 1)
     auto coef = [&](int k, float& c0, float& cgr, float& cgi) {
-        const float4 f = field[col + plane * (k == nz ? 0 : k)];
-        c0 = P.b0 * f.z;
-        const float bgr = P.bg * f.z;
-        cgr = bgr * f.x;
-        cgi = bgr * f.y;
+        const int64_t o = col + plane * (k == nz ? 0 : k);
+        if (DENS) {
+            const float4 f = static_cast<const float4*>(field)[o];
+            c0 = P.b0 * f.z;
+            const float bgr = P.bg * f.z;
+            cgr = bgr * f.x;
+            cgi = bgr * f.y;
+        } else {
+            const float2 f = static_cast<const float2*>(field)[o];
+            c0 = P.b0;
+            cgr = P.bg * f.x;
+            cgi = P.bg * f.y;
+        }
     };
     float c0a = 0, cgra = 0, cgia = 0;
     if (nz >= 2) coef(0, c0a, cgra, cgia);
@@ -1456,6 +1553,8 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     F.e_rc = (float)(k2 * p->rc2);
     F.e_band = (float)(std::fabs(k2 * p->rc2) * 1.0e-5);
     F.w_scale = (float)std::exp2((double)F.e_rc);
+    F.m2_min = (float)(1.0e-32 / ((double)F.w_scale * (double)F.w_scale));
+    F.inv_k2 = (float)(1.0 / k2);
     F.rc_cull2 = (float)(p->rc2 * (1.0 + 1.0e-4));
     F.cellf[0] = (float)p->a_col[0]; F.cellf[1] = (float)p->a_col[4]; F.cellf[2] = (float)p->a_col[8];
     // the unwrapped cell range of a tile must fit the per-tile cell table
@@ -1599,16 +1698,18 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
 }
 
 template <int NP>
-static void launch_int_fast(const FastIntParams& P, const float4* field, double* img, int acc, double inv,
-                            hipStream_t s)
+static void launch_int_fast(const FastIntParams& P, bool dens, const void* field, double* img, int acc,
+                            double inv, hipStream_t s)
 {
     const int64_t plane = (int64_t)P.Mx * P.My;
-    hipLaunchKernelGGL(k_integrate_fast<NP>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
-                       field, img, acc, inv);
+    const dim3 grid((unsigned)((plane + 255) / 256));
+    if (dens) hipLaunchKernelGGL((k_integrate_fast<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+    else hipLaunchKernelGGL((k_integrate_fast<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
 }
 
 // FAST integration of all precession points, in passes of up to 32.
-static int launch_integrate_fast(const d2bi_plan* p, const float4* f, double* d_img, hipStream_t s)
+// dens: the field holds float4 {x, rho} (density field on), else float2 x.
+static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, double* d_img, hipStream_t s)
 {
     const double inv = 1.0 / p->nPrec;
     for (int p0 = 0, first = 1; p0 < p->nPrec; first = 0) {
@@ -1619,12 +1720,12 @@ static int launch_integrate_fast(const d2bi_plan* p, const float4* f, double* d_
         P.b0 = (float)p->b0; P.bg = (float)p->bg; P.dz = (float)p->dz;
         for (int i = 0; i < 32; i++) P.aa[i] = i < np ? (float)p->aa[p0 + i] : 0.0f;
         switch (np) {
-            case 32: launch_int_fast<32>(P, f, d_img, !first, inv, s); break;
-            case 16: launch_int_fast<16>(P, f, d_img, !first, inv, s); break;
-            case 8: launch_int_fast<8>(P, f, d_img, !first, inv, s); break;
-            case 4: launch_int_fast<4>(P, f, d_img, !first, inv, s); break;
-            case 2: launch_int_fast<2>(P, f, d_img, !first, inv, s); break;
-            default: launch_int_fast<1>(P, f, d_img, !first, inv, s); break;
+            case 32: launch_int_fast<32>(P, dens, f, d_img, !first, inv, s); break;
+            case 16: launch_int_fast<16>(P, dens, f, d_img, !first, inv, s); break;
+            case 8: launch_int_fast<8>(P, dens, f, d_img, !first, inv, s); break;
+            case 4: launch_int_fast<4>(P, dens, f, d_img, !first, inv, s); break;
+            case 2: launch_int_fast<2>(P, dens, f, d_img, !first, inv, s); break;
+            default: launch_int_fast<1>(P, dens, f, d_img, !first, inv, s); break;
         }
         HIPCHK(hipGetLastError());
         p0 += np;
@@ -1643,7 +1744,7 @@ static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double*
     return D2BI_OK;
 }
 
-static int run_integrate(const d2bi_plan* p, int mode, hipStream_t s, Timer* T = nullptr)
+static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s, Timer* T = nullptr)
 {
     Workspace& w = g.ws;
     const int64_t plane = (int64_t)p->Mx * p->My;
@@ -1660,7 +1761,7 @@ static int run_integrate(const d2bi_plan* p, int mode, hipStream_t s, Timer* T =
         return D2BI_OK;
     }
     if (T) CHK(T->record(4));
-    CHK(launch_integrate_fast(p, w.f32_field.as<float4>(), d_img, s));
+    CHK(launch_integrate_fast(p, dens, w.f32_field.p, d_img, s));
     if (T) CHK(T->record(5));
     return D2BI_OK;
 }
@@ -1686,7 +1787,7 @@ static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, d
     CHK(T.record(1));
     int64_t ng = 0;
     CHK(run_voxel(r, &plan, tab, ax, mode, false, st ? &ng : nullptr, s, &T));
-    CHK(run_integrate(&plan, mode, s, &T));
+    CHK(run_integrate(&plan, r->use_density != 0, mode, s, &T));
     CHK(T.record(6));
     HIPCHK(hipStreamSynchronize(s));
     if (st) {
@@ -1750,7 +1851,7 @@ struct Cond {
 };
 
 struct Batch {
-    Buf mask, blkcnt, blkoff, axc, axz, axs, img, aa, guard, mpc;
+    Buf mask, blkcnt, blkoff, axc, axz, axs, axq, img, aa, guard, mpc;
     std::vector<unsigned char> key;
     uint64_t gen = 0;
     hipGraphExec_t exec = nullptr;
@@ -1835,7 +1936,7 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
                        gc, s);
     HIPCHK(hipGetLastError());
-    return launch_integrate_fast(p, w.f32_field.as<float4>(), d_img, s);
+    return launch_integrate_fast(p, c.dens, w.f32_field.p, d_img, s);
 }
 
 // Batched conditions.  The first call with a given set of inputs plans every
@@ -1896,6 +1997,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         CHK(B.axc.ensure(sizeof(int) * o_ax));
         CHK(B.axz.ensure(sizeof(double) * o_ax));
         CHK(B.axs.ensure(sizeof(int) * o_axs));
+        CHK(B.axq.ensure(sizeof(QL) * o_ax));
         CHK(B.img.ensure(sizeof(double) * o_img));
         CHK(B.aa.ensure(sizeof(double) * std::max<int64_t>(o_aa, 1)));
         CHK(B.guard.ensure(sizeof(unsigned long long) * nRec));
@@ -1905,6 +2007,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         {
             std::vector<int> hc(o_ax), hs(o_axs);
             std::vector<double> hz(o_ax), haa(std::max<int64_t>(o_aa, 1));
+            std::vector<QL> hq(o_ax);
             for (int i = 0; i < nRec; i++) {
                 Cond& c = C[i];
                 int64_t oc = c.o_ax, os = c.o_axs;
@@ -1912,6 +2015,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
                     std::copy(c.tab.cell[a].begin(), c.tab.cell[a].end(), hc.begin() + oc);
                     std::copy(c.tab.zi[a].begin(), c.tab.zi[a].end(), hz.begin() + oc);
                     std::copy(c.tab.start[a].begin(), c.tab.start[a].end(), hs.begin() + os);
+                    std::copy(c.tab.ql[a].begin(), c.tab.ql[a].end(), hq.begin() + oc);
+                    c.ax.ql[a] = B.axq.as<QL>() + oc;
                     c.ax.cell[a] = B.axc.as<int>() + oc;
                     c.ax.zi[a] = B.axz.as<double>() + oc;
                     c.ax.start[a] = B.axs.as<int>() + os;
@@ -1923,6 +2028,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             HIPCHK(hipMemcpy(B.axc.p, hc.data(), sizeof(int) * o_ax, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(B.axz.p, hz.data(), sizeof(double) * o_ax, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(B.axs.p, hs.data(), sizeof(int) * o_axs, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.axq.p, hq.data(), sizeof(QL) * o_ax, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(B.aa.p, haa.data(), sizeof(double) * haa.size(), hipMemcpyHostToDevice));
         }
         // counting pass: sizes of every condition

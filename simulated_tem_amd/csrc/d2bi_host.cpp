@@ -10,6 +10,7 @@
 // membership decision the kernels make.
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <algorithm>
 #include <thread>
@@ -30,6 +31,12 @@ static const double HP = 4.135667696;
 static const double HBAR = HP / (2 * 3.14159265390);
 // huge(1.0) of a default real, as a double
 static const double HUGE_R4 = 3.4028234663852886e38;
+
+#define HCHK(x)                      \
+    do {                             \
+        const int rc_ = (x);         \
+        if (rc_ < 0) return rc_;     \
+    } while (0)
 
 static inline double& m3(double* m, int i, int j) { return m[3 * (j - 1) + (i - 1)]; }
 static inline double m3(const double* m, int i, int j) { return m[3 * (j - 1) + (i - 1)]; }
@@ -387,6 +394,582 @@ int setup_tilt_ur(d2bi_record* r, const double* U, const double* R, int mxy, int
     return D2BI_OK;
 }
 
+// ------------------------------------------------------------------ (g, ng) tilt search
+//
+// setSupercellTilt1a / setSupercellTilt1 without atom positions
+// (Lib_DynamicalTwoBeamImaging.f90:795-1045): find the crystal rotation U and
+// the stage tilt R that put the (ng g) reflection at the Bragg condition with
+// the weakest excitation of every other reflection (findBestRotationMatrix,
+// Lib_DiffractionConditions.f90:129-357), then tilt about y to the dark-field
+// minimum of |phi_g(L)|^2 (tiltToDarkField, :751-927).  A fractional ng
+// interpolates the tilts of floor(ng) and ceiling(ng) by quaternion slerp.
+
+// Lib_DiffractionConditions.f90:24-29
+static const int NTHETA = 2000;
+static const double RHO_MAX = 10.0;
+static const double THETA_MAX = 5.0;                       // degrees
+static const double DEGTORAD = 3.141592654 / 180;
+// Lib_Lattices.f90:65-69, 100-102
+enum { LATT_SC = 1, LATT_BCC = 2, LATT_FCC = 3, LATT_HCP = 4 };
+enum { SYM_CUBIC = 1, SYM_HEX = 2 };
+
+// Fortran NINT: nearest integer, halves away from zero
+static int nint_away(double v) { return (int)std::lround(v); }
+
+// Fortran MOD for integers (sign of the dividend), as C's %.
+static int fmod_i(int a, int b) { return a % b; }
+
+// permittedReflections (Lib_Lattices.f90:678-735): [hkl] with
+// h^2 + k^2 + l^2 <= rho^2 (hcp: its own metric) allowed by the lattice,
+// l outer, k, h inner.
+static void permitted_reflections(int latt, double rho, std::vector<int>& hkl)
+{
+    const int mm = (int)std::ceil(rho + 1.0e-12);
+    hkl.clear();
+    for (int lp = -mm; lp <= mm; lp++)
+        for (int kp = -mm; kp <= mm; kp++)
+            for (int hp = -mm; hp <= mm; hp++) {
+                int rr = lp * lp + kp * kp + hp * hp;
+                bool ok;
+                switch (latt) {
+                    case LATT_FCC: {
+                        if (rr > rho * rho + 1.0e-6) continue;
+                        int hx = fmod_i(hp, 2), kx = fmod_i(kp, 2), lx = fmod_i(lp, 2);
+                        ok = (hx * hx + kx * kx + lx * lx) - (hx * kx + kx * lx + lx * hx) == 0;
+                        break;
+                    }
+                    case LATT_BCC:
+                        if (rr > rho * rho + 1.0e-6) continue;
+                        ok = fmod_i(hp + kp + lp, 2) == 0;
+                        break;
+                    case LATT_HCP:
+                        // (hp-kp)^2/3 is an integer division in the reference
+                        rr = (int)std::floor((double)((hp + kp) * (hp + kp) + (hp - kp) * (hp - kp) / 3) +
+                                             lp * lp * 0.375);
+                        if (rr > rho * rho + 1.0e-6) continue;
+                        ok = !((fmod_i(hp + 2 * kp, 3) == 0) && (fmod_i(std::abs(lp), 2) == 1));
+                        break;
+                    default:
+                        if (rr > rho * rho + 1.0e-6) continue;
+                        ok = true;
+                }
+                if (ok) { hkl.push_back(hp); hkl.push_back(kp); hkl.push_back(lp); }
+            }
+}
+
+// find_gvectors_permittedReflections (Lib_DiffractionConditions.f90:1727-1789):
+// g-vectors of the permitted reflections; [000] is swapped to the first
+// place and [hkl] to the second.  The second swap exchanges the index
+// triples of places 1 and ihkl (not 2 and ihkl) in the reference; the
+// extinction distances are looked up from these triples, so the same
+// exchange is made here.
+static void permitted_gvectors(int latt, const double* B, const int* hkl, double rho, std::vector<double>& g,
+                               std::vector<int>& hp)
+{
+    permitted_reflections(latt, rho, hp);
+    const int n = (int)hp.size() / 3;
+    g.assign(3 * (size_t)n, 0.0);
+    int i000 = 0, ihkl = 0;                 // 1-based, 0 = absent
+    for (int ii = 1; ii <= n; ii++) {
+        const int* h = &hp[3 * (ii - 1)];
+        if (h[0] * h[0] + h[1] * h[1] + h[2] * h[2] == 0) i000 = ii;
+        int d0 = h[0] - hkl[0], d1 = h[1] - hkl[1], d2 = h[2] - hkl[2];
+        if (d0 * d0 + d1 * d1 + d2 * d2 == 0) ihkl = ii;
+        double hd[3] = {(double)h[0], (double)h[1], (double)h[2]};
+        reflection(B, hd, &g[3 * (ii - 1)]);
+    }
+    auto swap_g = [&](int a, int b) { for (int k = 0; k < 3; k++) std::swap(g[3 * (a - 1) + k], g[3 * (b - 1) + k]); };
+    auto swap_h = [&](int a, int b) { for (int k = 0; k < 3; k++) std::swap(hp[3 * (a - 1) + k], hp[3 * (b - 1) + k]); };
+    if (i000 != 0) { swap_g(1, i000); swap_h(1, i000); }
+    if (ihkl != 0) { swap_g(2, ihkl); swap_h(1, ihkl); }
+}
+
+// symmetry (Lib_ReadExtinctionDistances.f90:111-170): standard form of a
+// reflection under the lattice symmetry.
+static void standard_reflection(const int* in, int symm, int* out)
+{
+    int hh = std::abs(in[0]), kk = std::abs(in[1]), ll = std::abs(in[2]);
+    if (symm == SYM_CUBIC) {
+        if (hh >= std::max(kk, ll)) {
+            if (kk >= ll) { out[0] = hh; out[1] = kk; out[2] = ll; }
+            else          { out[0] = hh; out[1] = ll; out[2] = kk; }
+        } else if (kk >= std::max(hh, ll)) {
+            if (hh >= ll) { out[0] = kk; out[1] = hh; out[2] = ll; }
+            else          { out[0] = kk; out[1] = ll; out[2] = hh; }
+        } else {
+            if (hh >= kk) { out[0] = ll; out[1] = hh; out[2] = kk; }
+            else          { out[0] = ll; out[1] = kk; out[2] = hh; }
+        }
+    } else if (symm == SYM_HEX) {
+        if (in[0] * in[1] >= 0) {
+            if (hh >= kk) { out[0] = hh; out[1] = kk; out[2] = ll; }
+            else          { out[0] = kk; out[1] = hh; out[2] = ll; }
+        } else {
+            if (hh >= kk) { out[0] = hh; out[1] = -kk; out[2] = ll; }
+            else          { out[0] = kk; out[1] = -hh; out[2] = ll; }
+        }
+    } else {
+        out[0] = in[0]; out[1] = in[1]; out[2] = in[2];
+    }
+}
+
+// readExtinctionDistance (Lib_ReadExtinctionDistances.f90:29-109) for a list
+// of reflections: two header lines, then up to 1000 records
+// "element h k l T V Re(U_g) Im(U_g) xi_g"; a reflection matches a record
+// when their standard forms agree.  Missing reflections keep huge(1.0)
+// (their beams are too weak to matter).  Returns whether all were found.
+static int read_xi_table(const char* filename, const std::vector<int>& hkl, int symm, std::vector<double>& xi,
+                         bool* all_found)
+{
+    const size_t n = hkl.size() / 3;
+    xi.assign(n, HUGE_R4);
+    std::vector<char> found(n, 0);
+    std::vector<int> hs(3 * n);
+    for (size_t j = 0; j < n; j++) standard_reflection(&hkl[3 * j], symm, &hs[3 * j]);
+    FILE* f = std::fopen(filename, "r");
+    if (!f) return seterr(D2BI_ERR_ARG, "extinction distance file \"%s\" not found", filename);
+    char line[512];
+    int ok = 1;
+    for (int i = 0; i < 2 && ok; i++) ok = std::fgets(line, sizeof line, f) != nullptr;
+    if (!ok) { std::fclose(f); return seterr(D2BI_ERR_ARG, "cannot read the header of \"%s\"", filename); }
+    size_t nfound = 0;
+    for (int rec = 0; rec < 1000 && nfound < n; rec++) {
+        if (!std::fgets(line, sizeof line, f)) break;
+        char name[64];
+        int id[3];
+        double rd[5];
+        if (std::sscanf(line, "%63s %d %d %d %lf %lf %lf %lf %lf", name, &id[0], &id[1], &id[2], &rd[0], &rd[1],
+                        &rd[2], &rd[3], &rd[4]) != 9)
+            break;                              // end of the table / another format
+        int is[3];
+        standard_reflection(id, symm, is);
+        for (size_t j = 0; j < n; j++)
+            if (is[0] == hs[3 * j] && is[1] == hs[3 * j + 1] && is[2] == hs[3 * j + 2]) {
+                if (!found[j]) nfound++;
+                found[j] = 1;
+                xi[j] = rd[4];
+            }
+    }
+    std::fclose(f);
+    *all_found = nfound == n;
+    return D2BI_OK;
+}
+
+// intensityAtDepthZ (Lib_DiffractionConditions.f90:1209-1258): |phi_g(z)|^2
+// in a perfect crystal from the two Bloch-wave eigenvalues.
+static double intensity_at_depth(double xi0, double xig, double sg, double z)
+{
+    const double b0 = PI / xi0, bg = PI / xig, aa = 2 * PI * sg;
+    const double dd = std::sqrt(aa * aa / 4 + bg * bg);
+    const double l1 = (b0 + aa / 2) + dd, l2 = (b0 + aa / 2) - dd;
+    double re = std::cos(l1 * z) - std::cos(l2 * z);
+    double im = std::sin(l1 * z) - std::sin(l2 * z);
+    const double f = bg / (2 * dd);
+    re = re * f;
+    im = im * f;
+    return re * re - im * (-im);
+}
+
+// findRotationsForBraggCondition2 (Lib_DiffractionConditions.f90:1382-1518):
+// azimuths phi (rotation about y after theta about x) putting g at the
+// Bragg condition.  limit: 0 = |phi - phi0| <= THETA_MAX, 1 = twice that,
+// 2 = no limit.
+static int bragg_azimuths(double v, const double* g, double cosq, double sinq, double phi0, int limit, double* phi)
+{
+    int nphi = 0;
+    const double xx = g[1] * sinq + g[2] * cosq;
+    double aa = (g[0] * g[0] + g[1] * g[1]) + g[2] * g[2];
+    aa = -((aa * HBAR) / ((2 * ME) * v));
+    double uu = g[0] * g[0] + xx * xx;
+    double vv = uu - aa * aa;
+    if (vv < 0) return 0;
+    uu = 1 / uu;
+    vv = (std::sqrt(vv) * g[0]) * uu;
+    uu = (aa * xx) * uu;
+    const double lim = limit == 0 ? THETA_MAX * DEGTORAD : 2 * THETA_MAX * DEGTORAD;
+    for (int branch = 0; branch < 2; branch++) {
+        const double c = branch == 0 ? uu + vv : uu - vv;
+        if (std::fabs(c) <= 1) {
+            double pp = std::acos(c);
+            const int jj = nint_away((pp - phi0) / (2 * PI));
+            pp = pp - 2 * jj * PI;
+            if (limit == 2 || std::fabs(pp - phi0) <= lim) {
+                phi[nphi] = pp;
+                phi[nphi + 1] = -pp;
+                nphi += 2;
+            }
+        }
+    }
+    return nphi;
+}
+
+// findAllEulerAngles (Lib_DiffractionConditions.f90:1565-1623): incident
+// directions khat = (-sin phi, sin theta cos phi, cos theta cos phi) at
+// the Bragg condition for theta on a grid of NTHETA steps.
+static void all_euler_angles(double v, const double* g, double theta0, double phi0, double theta_max, int limit,
+                             std::vector<double>& khat)
+{
+    khat.clear();
+    const double theta_min = std::max(0.0, theta0 - theta_max / 2);
+    for (int ii = 1; ii <= NTHETA; ii++) {
+        const double theta = theta_min + theta_max * ii / NTHETA;
+        const double cost = std::cos(theta), sint = std::sin(theta);
+        double phi[4];
+        const int nphi = bragg_azimuths(v, g, cost, sint, phi0, limit, phi);
+        for (int jj = 0; jj < nphi; jj++) {
+            const double cosp = std::cos(phi[jj]), sinp = std::sin(phi[jj]);
+            khat.push_back(-sinp);
+            khat.push_back(sint * cosp);
+            khat.push_back(cost * cosp);
+        }
+    }
+}
+
+// findBestRotationMatrix (Lib_DiffractionConditions.f90:129-357) without the
+// near-Bragg list: U from rotateToStandardFrame, R the tilt whose incident
+// direction minimises the summed two-beam intensity of every other
+// permitted reflection (scoreForIncidentVector1/2, :1835-1889).
+static int best_rotation(double v, const double* z0, const double* A_cell, int latt, const int* hkl,
+                         const char* xifile, double foil, double* U, double* R)
+{
+    double BB[9], gg[3], ghkl0[3];
+    recip(A_cell, BB);
+    const double hd[3] = {(double)hkl[0], (double)hkl[1], (double)hkl[2]};
+    reflection(BB, hd, gg);
+    rotate_to_standard_frame(z0, A_cell, hd, U);
+    for (int i = 1; i <= 3; i++) ghkl0[i - 1] = (m3(U,i,1) * gg[0] + m3(U,i,2) * gg[1]) + m3(U,i,3) * gg[2];
+    std::vector<double> pg;
+    std::vector<int> ph;
+    permitted_gvectors(latt, BB, hkl, RHO_MAX, pg, ph);
+    const int nn = (int)ph.size() / 3;
+    std::vector<double> xi;
+    bool all_found = false;
+    HCHK(read_xi_table(xifile, ph, latt == LATT_HCP ? SYM_HEX : SYM_CUBIC, xi, &all_found));
+    std::vector<double> khat;
+    all_euler_angles(v, ghkl0, 0.0, 0.0, THETA_MAX * DEGTORAD, 0, khat);
+    if (khat.empty()) all_euler_angles(v, ghkl0, 0.0, 0.0, 2 * THETA_MAX * DEGTORAD, 0, khat);
+    if (khat.empty()) all_euler_angles(v, ghkl0, 0.0, 0.0, THETA_MAX * DEGTORAD, 2, khat);
+    const int nk = (int)khat.size() / 3;
+    if (nk == 0) return seterr(D2BI_ERR_ARG, "no tilt brings [%d %d %d] to the Bragg condition", hkl[0], hkl[1], hkl[2]);
+    // scores in parallel over incident directions; each is the reference's
+    // serial sum over reflections 3..nn
+    std::vector<double> score(nk);
+    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    auto work = [&](int t) {
+        for (int ii = t; ii < nk; ii += nth) {
+            const double* k = &khat[3 * ii];
+            double s = 0;
+            for (int j = 3; j <= nn; j++) {
+                const double eps = energy_deviation(v, k, &pg[3 * (j - 1)]);
+                s = s + intensity_at_depth(xi[0], xi[j - 1], e2l(v, eps), foil);
+            }
+            score[ii] = s;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    double best = HUGE_R4;
+    int ib = -1;
+    for (int ii = 0; ii < nk; ii++)
+        if (score[ii] < best) { best = score[ii]; ib = ii; }
+    if (ib < 0) return seterr(D2BI_ERR_ARG, "tilt search found no finite score");
+    const double* k = &khat[3 * ib];
+    const double phi = -std::asin(k[0]);
+    const double theta = std::acos(k[2] / std::cos(phi));
+    const double ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+    m3(R,1,1) = cp;   m3(R,1,2) = st * sp; m3(R,1,3) = ct * sp;
+    m3(R,2,1) = 0.0;  m3(R,2,2) = ct;      m3(R,2,3) = -st;
+    m3(R,3,1) = -sp;  m3(R,3,2) = st * cp; m3(R,3,3) = ct * cp;
+    if (m3(R,3,3) < 0)
+        for (int i = 1; i <= 3; i++) { m3(R,i,3) = -m3(R,i,3); m3(R,i,2) = -m3(R,i,2); }
+    return D2BI_OK;
+}
+
+// GoldenSection (Lib_GoldenSection.f90): bracket, then golden-section
+// minimisation with the reference's tolerances and step limits.
+namespace {
+struct Golden {
+    double x1, x2, x3, y1, y2, y3, xtol, ytol;
+    bool bounded = false;
+    int step = 0;
+    static constexpr double TOL = 1.0e-6, MINSTEP = 50, BOUNDSTEP = 10;
+    double gold() const { return (1 + std::sqrt(5.0)) / 2; }
+    Golden(double a, double b, double ya, double yb)
+    {
+        if (a < b) { x1 = a; x3 = b; y1 = ya; y3 = yb; }
+        else       { x3 = b; x1 = a; y3 = yb; y1 = ya; }
+        x2 = 0.0;
+        y2 = HUGE_R4;
+        xtol = (x3 - x1) * TOL;
+        ytol = std::max(TOL, std::fabs(y3 - y1)) * TOL;
+    }
+    double next() const
+    {
+        const double G = gold();
+        if (!bounded) {
+            if (step == 0) return x1 + (x3 - x1) * (G - 1);
+            if (y1 < y3) return x1 - (x3 - x1) / G;
+            return x3 + (x3 - x1) / G;
+        }
+        return x1 + (x3 - x2);
+    }
+    void check_bounds(double xt, double yt)
+    {
+        if (step == 1) {
+            if (yt < std::min(y1, y3)) { bounded = true; x2 = xt; y2 = yt; }
+            else if ((yt > y1) && (y1 > y3)) { bounded = false; x1 = xt; y1 = yt; }
+            else if ((yt > y3) && (y3 > y1)) { bounded = false; x3 = xt; y3 = yt; }
+            else if ((y1 > yt) && (yt > y3)) { bounded = false; x1 = xt; y1 = yt; }
+            else if ((y3 > yt) && (yt > y1)) { bounded = false; x3 = xt; y3 = yt; }
+            return;
+        }
+        if (y1 < y3) {
+            bounded = !(yt < y1);
+            x2 = x1; x1 = xt; y2 = y1; y1 = yt;
+        } else {
+            bounded = !(yt < y3);
+            x2 = x3; x3 = xt; y2 = y3; y3 = yt;
+        }
+    }
+    // returns converged; *within = false when the step limit is reached
+    bool minimise(double& xt, double& yt, bool* within)
+    {
+        step++;
+        if (!bounded) {
+            check_bounds(xt, yt);
+            *within = bounded || (step <= BOUNDSTEP);
+            return false;
+        }
+        *within = step <= MINSTEP;
+        if (!*within) return false;
+        if (yt < y2) {
+            if (xt < x2) { x3 = x2; y3 = y2; x2 = xt; y2 = yt; }
+            else         { x1 = x2; y1 = y2; x2 = xt; y2 = yt; }
+        } else {
+            if (xt > x2) { x3 = xt; y3 = yt; }
+            else         { x1 = xt; y1 = yt; }
+        }
+        const bool is = (std::fabs(x3 - x1) < xtol) || (std::fabs(y2 - std::max(y1, y3)) < ytol);
+        if (is) { xt = x2; yt = y2; }
+        return is;
+    }
+};
+}  // namespace
+
+// tiltToDarkField (Lib_DiffractionConditions.f90:751-927): tilt about y by
+// the angle in +-THETA_MAX/100 degrees that minimises the perfect-crystal
+// |phi_g(L)|^2, if that is below its untilted value; R = Ry R.
+static void tilt_to_dark_field(double v, const double* A_cell, const double* hkl, double xi0, double xig,
+                               double foil, const double* U, double* R)
+{
+    (void)xi0;
+    double BB[9], kk[3], RU[9], gg[3];
+    recip(A_cell, BB);
+    reflection(BB, hkl, kk);
+    matmul3(R, U, RU);
+    for (int i = 1; i <= 3; i++) gg[i - 1] = (m3(RU,i,1) * kk[0] + m3(RU,i,2) * kk[1]) + m3(RU,i,3) * kk[2];
+    const double bg = 1 / xig;
+    static const double khat[3] = {0.0, 0.0, 1.0};
+    const double sg = e2l(v, energy_deviation(v, khat, gg));
+    auto phiL2 = [&](double s, double L) {
+        const double t = bg * std::sin((PI * std::sqrt(s * s + bg * bg)) * L);
+        return (t * t) / (s * s + bg * bg);
+    };
+    const double y0 = phiL2(sg, foil);
+    auto tilted = [&](double x, double& cost, double& sint) {
+        cost = std::cos(x); sint = std::sin(x);
+        const double Lp = foil / cost;
+        const double sgp = (sint * gg[0] - cost * gg[2]) / (2 * PI);
+        return phiL2(sgp, Lp);
+    };
+    double cost, sint;
+    const double x1 = -((THETA_MAX * 0.01) * DEGTORAD), x3 = (THETA_MAX * 0.01) * DEGTORAD;
+    const double y1 = tilted(x1, cost, sint), y3 = tilted(x3, cost, sint);
+    Golden gold(x1, x3, y1, y3);
+    double x2, y2;
+    for (;;) {
+        x2 = gold.next();
+        y2 = tilted(x2, cost, sint);
+        bool within;
+        const bool conv = gold.minimise(x2, y2, &within);
+        if (conv || !within) break;
+    }
+    if (y2 < y0) { cost = std::cos(x2); sint = std::sin(x2); }
+    else         { cost = 1.0; sint = 0.0; }
+    double RR[9] = {0};
+    m3(RR,1,1) = cost;  m3(RR,1,3) = sint;
+    m3(RR,2,2) = 1.0;
+    m3(RR,3,1) = -sint; m3(RR,3,3) = cost;
+    matmul3(RR, R, R);
+}
+
+// Quaternions (Lib_Quaternions.f90): q = (w, x, y, z).
+namespace {
+struct Quat { double w, x, y, z; };
+}
+
+// Eigen-decomposition of a symmetric 4x4 matrix by cyclic Jacobi rotations;
+// returns the unit eigenvector of the largest eigenvalue.  This stands in
+// for the LAPACK DSYEV call of RotMatToQuaternion.
+static void sym4_top_eigenvector(const double K[4][4], double v[4])
+{
+    double A[4][4], V[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) { A[i][j] = K[i][j]; V[i][j] = i == j ? 1.0 : 0.0; }
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < 4; p++)
+            for (int q = p + 1; q < 4; q++) off += A[p][q] * A[p][q];
+        if (off < 1.0e-300) break;
+        for (int p = 0; p < 4; p++)
+            for (int q = p + 1; q < 4; q++) {
+                if (A[p][q] == 0.0) continue;
+                const double th = (A[q][q] - A[p][p]) / (2 * A[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 4; k++) {               // A <- A J
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 4; k++) {               // A <- J^T A
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 4; k++) {               // V <- V J
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int top = 0;
+    for (int i = 1; i < 4; i++)
+        if (A[i][i] > A[top][top]) top = i;
+    double n = 0.0;
+    for (int k = 0; k < 4; k++) n += V[k][top] * V[k][top];
+    n = 1 / std::sqrt(n);
+    for (int k = 0; k < 4; k++) v[k] = V[k][top] * n;
+}
+
+// RotMatToQuaternion (Lib_Quaternions.f90:246-288): the eigenvector of the
+// largest eigenvalue of the symmetric matrix built from R, with the sign
+// fixed so its last component is not positive.
+static Quat quat_from_rot(const double* R)
+{
+    double K[4][4] = {{0}};
+    K[0][0] = m3(R,1,1) - m3(R,2,2) - m3(R,3,3);
+    K[1][0] = m3(R,2,1) + m3(R,1,2);
+    K[2][0] = m3(R,3,1) + m3(R,1,3);
+    K[3][0] = m3(R,2,3) - m3(R,3,2);
+    K[1][1] = m3(R,2,2) - m3(R,1,1) - m3(R,3,3);
+    K[2][1] = m3(R,3,2) + m3(R,2,3);
+    K[3][1] = m3(R,3,1) - m3(R,1,3);
+    K[2][2] = m3(R,3,3) - m3(R,1,1) - m3(R,2,2);
+    K[3][2] = m3(R,1,2) - m3(R,2,1);
+    K[3][3] = m3(R,1,1) + m3(R,2,2) + m3(R,3,3);
+    for (int i = 0; i < 4; i++)                          // lower triangle is what is given
+        for (int j = i + 1; j < 4; j++) K[i][j] = K[j][i];
+    double v[4];
+    sym4_top_eigenvector(K, v);
+    if (v[3] > 0) for (int k = 0; k < 4; k++) v[k] = -v[k];
+    return Quat{-v[3], v[0], v[1], v[2]};
+}
+
+// quaternionToRotMat (Lib_Quaternions.f90:290-316)
+static void rot_from_quat(const Quat& q, double* R)
+{
+    double ss = (((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w) / 2;
+    m3(R,1,1) = ss - (q.y * q.y + q.z * q.z);
+    m3(R,2,1) = (q.x * q.y + q.z * q.w);
+    m3(R,3,1) = (q.x * q.z - q.y * q.w);
+    m3(R,1,2) = (q.x * q.y - q.z * q.w);
+    m3(R,2,2) = ss - (q.x * q.x + q.z * q.z);
+    m3(R,3,2) = (q.y * q.z + q.x * q.w);
+    m3(R,1,3) = (q.x * q.z + q.y * q.w);
+    m3(R,2,3) = (q.y * q.z - q.x * q.w);
+    m3(R,3,3) = ss - (q.x * q.x + q.y * q.y);
+    ss = 1 / ss;
+    for (int i = 0; i < 9; i++) R[i] = R[i] * ss;
+}
+
+// slerp (Lib_Quaternions.f90:526-564), with midpoint (:504-524) and
+// normalise (:203-219)
+static Quat slerp(const Quat& a, const Quat& b, double x)
+{
+    double x1 = ((a.w * b.w + a.x * b.x) + a.y * b.y) + a.z * b.z;
+    Quat r;
+    if (x1 >= 1.0) return a;
+    if (x1 <= -1.0) {
+        r = Quat{a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z};
+        double dd = ((r.w * r.w + r.x * r.x) + r.y * r.y) + r.z * r.z;
+        if (dd > 0) {
+            dd = 1 / std::sqrt(dd);
+            r.w *= dd; r.x *= dd; r.y *= dd; r.z *= dd;
+        }
+        return r;
+    }
+    double x2 = 2 * x1 * x1 - 1;
+    const double omega = std::acos(x2);
+    x2 = 1 / std::sin(omega);
+    x1 = std::sin((1 - x) * omega) * x2;
+    x2 = std::sin(x * omega) * x2;
+    r.w = x1 * a.w + x2 * b.w;
+    r.x = x1 * a.x + x2 * b.x;
+    r.y = x1 * a.y + x2 * b.y;
+    r.z = x1 * a.z + x2 * b.z;
+    double tt = ((r.w * r.w + r.x * r.x) + r.y * r.y) + r.z * r.z;
+    if (tt == 0) return Quat{1, 0, 0, 0};
+    tt = 1 / std::sqrt(tt);
+    r.w *= tt; r.x *= tt; r.y *= tt; r.z *= tt;
+    return r;
+}
+
+// setSupercellTilt1 without atom positions (Lib_DynamicalTwoBeamImaging.f90:795-939)
+// up to the tilt: U, R for the (ng g) condition; d[2] becomes the depth
+// after the tilt.
+static int tilt_for_ng(d2bi_record* r, const double* k, int ng, int latt, const char* xifile, double* d)
+{
+    const int hkl[3] = {nint_away(ng * r->g[0]), nint_away(ng * r->g[1]), nint_away(ng * r->g[2])};
+    const double foil = d[2];
+    const double vv = velocity(r->E * 1000);
+    HCHK(best_rotation(vv, k, r->a_cell, latt, hkl, xifile, std::fabs(foil), r->U, r->R));
+    double z = std::fabs(foil / m3(r->R,3,3));
+    const double g0[3] = {(double)nint_away(r->g[0]), (double)nint_away(r->g[1]), (double)nint_away(r->g[2])};
+    tilt_to_dark_field(velocity(r->E * 1000), r->a_cell, g0, r->xi0, r->xig, z, r->U, r->R);
+    d[2] = std::fabs(d[2] / m3(r->R,3,3));
+    return D2BI_OK;
+}
+
+// setSupercellTilt1a without atom positions (Lib_DynamicalTwoBeamImaging.f90:943-1045)
+int setup_tilt_ng(d2bi_record* r, const double* k, double ng, int latt, const char* xifile, int mxy, int mzz,
+                  double* d)
+{
+    if (!(ng >= 1.0) || ng > 1.0e6) return seterr(D2BI_ERR_ARG, "ng must be >= 1");
+    if (latt < LATT_SC || latt > LATT_HCP) return seterr(D2BI_ERR_ARG, "unknown lattice code %d", latt);
+    if (!xifile) return seterr(D2BI_ERR_ARG, "the (g,ng) tilt needs an extinction distance file");
+    const double dd[3] = {d[0], d[1], d[2]};
+    const int nf = (int)std::floor(ng);
+    double dw[3] = {dd[0], dd[1], dd[2]};
+    HCHK(tilt_for_ng(r, k, nf, latt, xifile, dw));
+    const Quat q1 = quat_from_rot(r->R);
+    Quat q2 = q1;
+    if (ng - nf > 1.0e-4) {
+        double dw2[3] = {dd[0], dd[1], dd[2]};
+        HCHK(tilt_for_ng(r, k, (int)std::ceil(ng), latt, xifile, dw2));
+        q2 = quat_from_rot(r->R);
+    }
+    rot_from_quat(slerp(q1, q2, ng - nf), r->R);
+    d[0] = dd[0];
+    d[1] = dd[1];
+    d[2] = std::fabs(dd[2] / m3(r->R,3,3));
+    finish_tilt(r, mxy, mzz, d);
+    return D2BI_OK;
+}
+
 // ------------------------------------------------------------------ synthetic crystals
 
 namespace {
@@ -507,6 +1090,14 @@ int d2bi_setup_tilt_ur(d2bi_record* rec, const double U[9], const double R[9], i
     if (!rec || !U || !R || !d || mxy < 1 || mzz < 1) return d2bi::seterr(D2BI_ERR_ARG, "bad argument");
     if (!(d[0] > 0 && d[1] > 0 && d[2] > 0)) return d2bi::seterr(D2BI_ERR_ARG, "voxel cuboid sides must be > 0");
     return d2bi::setup_tilt_ur(rec, U, R, mxy, mzz, d);
+}
+
+int d2bi_setup_tilt_ng(d2bi_record* rec, const double k[3], double ng, int32_t lattice, const char* xifile,
+                       int32_t mxy, int32_t mzz, double d[3])
+{
+    if (!rec || !k || !d || !xifile || mxy < 1 || mzz < 1) return d2bi::seterr(D2BI_ERR_ARG, "bad argument");
+    if (!(d[0] > 0 && d[1] > 0 && d[2] > 0)) return d2bi::seterr(D2BI_ERR_ARG, "voxel cuboid sides must be > 0");
+    return d2bi::setup_tilt_ng(rec, k, ng, lattice, xifile, mxy, mzz, d);
 }
 
 int d2bi_rotate_to_standard_frame(const double k[3], const double a_cell[9], const double hkl[3], double U[9])

@@ -20,6 +20,8 @@ int plan_image(const d2bi_record* r, int64_t nAtoms, double precAngle, int nPrec
 void rotate_to_standard_frame(const double* z0, const double* A, const double* hkl, double* U);
 int setup_tilt_sg(d2bi_record* r, const double* k, double sg, int mxy, int mzz, const double* d);
 int setup_tilt_ur(d2bi_record* r, const double* U, const double* R, int mxy, int mzz, const double* d);
+int setup_tilt_ng(d2bi_record* r, const double* k, double ng, int latt, const char* xifile, int mxy, int mzz,
+                  double* d);
 int synth_bcc(const d2bi_synth_params* p, double* x, int64_t cap, int64_t* nAtoms, double* box);
 
 }  // namespace d2bi

@@ -47,7 +47,7 @@ module d2bi_gpu_mod
     end type d2bi_stats
 
     public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_plan_image
-    public :: d2bi_setup_tilt_sg, d2bi_setup_tilt_ur, d2bi_rotate_to_standard_frame
+    public :: d2bi_setup_tilt_sg, d2bi_setup_tilt_ur, d2bi_setup_tilt_ng, d2bi_rotate_to_standard_frame
     public :: d2bi_gpu_atoms_upload, d2bi_gpu_atoms_free, d2bi_gpu_image_dev, d2bi_gpu_blur
     public :: d2bi_write_png16, d2bi_error_message
 
@@ -87,6 +87,17 @@ module d2bi_gpu_mod
             real(c_double), intent(in)       :: U(9), R(9)
             integer(c_int32_t), value        :: mxy, mzz
             real(c_double), intent(in)       :: d(3)
+        end function
+        integer(c_int) function d2bi_setup_tilt_ng(rec, k, ng, lattice, xifile, mxy, mzz, d) &
+                bind(c, name="d2bi_setup_tilt_ng")
+            import :: c_int, c_int32_t, c_double, c_char, d2bi_record
+            type(d2bi_record), intent(inout) :: rec
+            real(c_double), intent(in)       :: k(3)
+            real(c_double), value            :: ng
+            integer(c_int32_t), value        :: lattice
+            character(kind=c_char), intent(in) :: xifile(*)
+            integer(c_int32_t), value        :: mxy, mzz
+            real(c_double), intent(inout)    :: d(3)
         end function
         integer(c_int) function d2bi_rotate_to_standard_frame(k, a_cell, hkl, U) &
                 bind(c, name="d2bi_rotate_to_standard_frame")

@@ -7,8 +7,8 @@
 !
 !   Supported: -f -a0 -lattice (cubic) -o -g -k -sg -dark -V -nPrecAngle
 !   -precAngle -nTomoAngle -tomoAngle -density -M -xpad -ypad -zpad -d
-!   -addOffset -U -R -xifile -xi0 -xig -png -png_min -png_max -png_blur,
-!   plus -mode fp64|fast and -device n.  -ng (tilt search) and -alterg are
+!   -addOffset -U -R -xifile -xi0 -xig -ng -png -png_min -png_max -png_blur,
+!   plus -mode fp64|fast and -device n.  -alterg/-dp, -surf and hcp are
 !   not part of this build and stop with a message.
 program d2bi_host
     use iso_fortran_env, only: real64, int64, output_unit
@@ -27,13 +27,13 @@ program d2bi_host
     real(real64) :: a0in(3), a0, xi0 = 103.889565d0, xig = 207.533203d0, V = 200.0d0
     real(real64) :: precAngle = 0.005d0, tomoAngle = 20.0d0, png_min = 0.0d0, png_max = 0.0d0
     real(real64) :: png_blur = 2.50d0
-    real(real64) :: darr(4 * NDIFFMAX), g3a(3, NDIFFMAX), k3a(3, NDIFFMAX), sga(NDIFFMAX)
+    real(real64) :: darr(4 * NDIFFMAX), g3a(3, NDIFFMAX), k3a(3, NDIFFMAX), sga(NDIFFMAX), nga(NDIFFMAX), ng
     logical      :: darka(NDIFFMAX), bl(NDIFFMAX)
     real(real64) :: d0(3), addOffset(3), U(3, 3), R(3, 3), dat(12)
     integer      :: M(2) = [4, 4], nPrec = 1, nTomo = 1, device = 0, mode
     logical      :: useDensity = .false., xpad = .false., ypad = .false., zpad = .false.
     logical      :: opPng = .true., haveU = .false., haveR = .false.
-    integer      :: nDiff = 1, n, ii, nn, kk, rc, Mx, My, Mz
+    integer      :: nDiff = 1, n, ii, nn, kk, rc, Mx, My, Mz, latcode
     real(real64), allocatable :: x(:, :), img(:, :), blurred(:, :)
     real(c_float), allocatable :: pngbuf(:)
     real(real64) :: a_super(3, 3), a_vox(3, 3), yy(3), xmin(3), xmax(3), d(3), rot(3, 3)
@@ -50,7 +50,7 @@ program d2bi_host
     call cla_init()
     if (.not. cla_has("f") .or. .not. cla_has("a0") .or. .not. cla_has("g") .or. .not. cla_has("k")) then
         write(*, '(a)') "d2bi v"//trim(VERSION)//" - simulated TEM in the dynamical two-beam approximation (MI355X)"
-        write(*, '(a)') "usage: d2bi -f file -a0 a [-lattice bcc] -g h,k,l[,...] -k u,v,w [-sg s] [-o prefix]"
+        write(*, '(a)') "usage: d2bi -f file -a0 a [-lattice bcc] -g h,k,l[,...] -k u,v,w [-sg s | -ng n] [-o prefix]"
         write(*, '(a)') "       [-dark|-nodark] [-V keV] [-M xy[,z]] [-d x,y,z] [-U 9 reals] [-R 9 reals]"
         write(*, '(a)') "       [-xifile f | -xi0 a -xig b] [-density] [-nPrecAngle n -precAngle rad]"
         write(*, '(a)') "       [-nTomoAngle n -tomoAngle deg] [-xpad|-ypad|-zpad] [-png_blur A] [-mode fp64|fast]"
@@ -62,6 +62,11 @@ program d2bi_host
     if (n == 1) a0in(2:3) = a0in(1)
     call cla_string("lattice", latname)
     if (trim(latname) == "hcp") error stop "d2bi error - hcp lattices are not part of this build"
+    select case (trim(latname))
+        case ("bcc"); latcode = 2
+        case ("fcc"); latcode = 3
+        case default; latcode = 1
+    end select
     outfile0 = filename
     call cla_string("o", outfile0)
     bl(1) = opPng
@@ -76,7 +81,6 @@ program d2bi_host
     call cla_reals("xig", dat, n); if (n > 0) xig = dat(1)
     if (len_trim(xifile) > 0 .and. (cla_has("xi0") .or. cla_has("xig"))) &
         error stop "d2bi error - attempt to define xi_0 or xi_g and read from file"
-    if (cla_has("ng")) error stop "d2bi error - the (g,ng) tilt search is not part of this build; use -sg or -U/-R"
     if (cla_has("alterg") .or. cla_has("dp")) error stop "d2bi error - -alterg/-dp are not part of this build"
     ! g-vectors and zone axes (d2bi.f90:244-325)
     darr = 0
@@ -118,6 +122,22 @@ program d2bi_host
     else if (n * (n - nDiff) /= 0) then
         error stop "d2bi error - array of -sg deviation parameters must match k,g-vectors "
     end if
+    nga = UNSET
+    call cla_reals("ng", nga, n)
+    if (n == 1) then
+        nga(1:nDiff) = nga(1)
+    else if (nDiff == 1 .and. n > 1) then
+        nDiff = n
+        do ii = 2, nDiff
+            k3a(:, ii) = k3a(:, 1)
+            g3a(:, ii) = g3a(:, 1)
+        end do
+        sga(1:nDiff) = sga(1)
+    else if (n * (n - nDiff) /= 0) then
+        error stop "d2bi error - array of -ng parameters must match k,g-vectors "
+    end if
+    if (cla_has("ng") .and. cla_has("sg")) error stop "d2bi error - attempt to define both n_g and s_g"
+    if (cla_has("ng") .and. len_trim(xifile) == 0) error stop "d2bi error - the (g,ng) tilt search needs -xifile"
     darka = .true.
     call cla_logical("dark", darka, n)
     if (n == 1) darka(1:nDiff) = darka(1)
@@ -224,7 +244,14 @@ program d2bi_host
         rec = rec0
         rec%g = g3a(:, nn)
         sg = sga(nn)
+        ng = nga(nn)
         dark = darka(nn)
+        !   bright field with ng > 1 images the [hkl] point at ng = 1 (d2bi.f90:502-510)
+        if (.not. dark .and. ng /= UNSET .and. ng > 1) then
+            write(*, '(a)') "d2bi warning - conflicting options '-nodark/-bright' and '-ng > 1'"
+            write(*, '(a)') "   setting ng = 1."
+            ng = 1
+        end if
         write(*, '(a)') ""
         write(*, '(a,i4,a,i4)') " diffraction condition ", nn, "/", nDiff
         write(*, '(a,3f8.2)') "Miller index   g = ", rec%g
@@ -253,6 +280,15 @@ program d2bi_host
         if (any(d <= 0)) error stop "d2bi error - '-d 0' (image space from the supercell) is not part of this build"
         if (sg /= UNSET) then
             rc = d2bi_setup_tilt_sg(rec, k3a(:, nn), sg, int(M(1), c_int32_t), int(M(2), c_int32_t), d)
+        else if (ng /= UNSET) then
+            !   setSupercellTilt1a: the reference sets the table's extinction
+            !   distances only after the tilt (d2bi.f90:1265), so the search
+            !   sees the constructor's 103.9, 207.5 A
+            rec%xi0 = 103.9d0
+            rec%xig = 207.5d0
+            write(*, '(a,f8.3)') "d2bi info - setting (g,ng) condition and finding crystal rotation and tilt, ng = ", ng
+            rc = d2bi_setup_tilt_ng(rec, k3a(:, nn), ng, int(latcode, c_int32_t), trim(xifile)//c_null_char, &
+                                    int(M(1), c_int32_t), int(M(2), c_int32_t), d)
         else if (haveR) then
             if (.not. haveU) U = reshape([1, 0, 0, 0, 1, 0, 0, 0, 1], [3, 3])
             rc = d2bi_setup_tilt_ur(rec, reshape(U, [9]), reshape(R, [9]), int(M(1), c_int32_t), &

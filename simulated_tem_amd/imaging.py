@@ -88,12 +88,17 @@ def lattice_a0(a0in):
     return ((a[0] + a[1]) + a[2]) / 3
 
 
+# Lib_Lattices.f90:65-69
+LATTICE_CODES = {"sc": 1, "bcc": 2, "fcc": 3, "hcp": 4}
+
+
 class DynamicalTwoBeamImaging:
     """DynamicalTwoBeamImaging_ctor0(super,latt,a0,k,g,E,mxy,mzz) and its
     setters.  super is given as (cell vectors a[3x3] with columns as vectors,
-    repeats n[3]), as SimpleSupercell_ctor(a,Nx,Ny,Nz) holds it."""
+    repeats n[3]), as SimpleSupercell_ctor(a,Nx,Ny,Nz) holds it; latt is the
+    lattice name (it only matters to the (g,ng) tilt search)."""
 
-    def __init__(self, super_a, super_n, a0, k, g, E=200.0, mxy=4, mzz=4):
+    def __init__(self, super_a, super_n, a0, k, g, E=200.0, mxy=4, mzz=4, latt="bcc"):
         self.rec = D2biRecord()
         r = self.rec
         r.a0 = a0
@@ -111,6 +116,7 @@ class DynamicalTwoBeamImaging:
         self.mxy, self.mzz = int(mxy), int(mzz)
         self.sg_target = float("nan")
         self.d = np.zeros(3)
+        self.latt = LATTICE_CODES.get(str(latt).lower(), 1)
 
     # --- setters (Lib_DynamicalTwoBeamImaging.f90:245-345)
     def setConventionalUnitCell(self, a_cell):
@@ -131,12 +137,21 @@ class DynamicalTwoBeamImaging:
     def setUseDensityField(self, use):
         self.rec.use_density = 1 if use else 0
 
-    def setSupercellTilt(self, U=None, R=None):
-        """setSupercellTilt0 (no arguments: tilt for the deviation parameter)
-        or setSupercellTilt2(U,R) (previously determined rotation)."""
+    def setSupercellTilt(self, U=None, R=None, ng=None, extinction_distances_filename=None):
+        """setSupercellTilt0 (no arguments: tilt for the deviation parameter),
+        setSupercellTilt2(U,R) (previously determined rotation) or
+        setSupercellTilt1a(ng, extinction_distances_filename) (the (g,ng)
+        tilt search, without atom positions).  The search uses the
+        extinction distances the record holds at the time (the d2bi driver
+        sets the table values only afterwards, d2bi.f90:1265)."""
         L = abi.load()
         d = (C.c_double * 3)(*self.d)
-        if U is None and R is None:
+        if ng is not None:
+            k = (C.c_double * 3)(*self.k)
+            check(L.d2bi_setup_tilt_ng(C.byref(self.rec), k, float(ng), self.latt,
+                                       str(extinction_distances_filename).encode(), self.mxy, self.mzz, d))
+            self.d = np.array(list(d))
+        elif U is None and R is None:
             k = (C.c_double * 3)(*self.k)
             check(L.d2bi_setup_tilt_sg(C.byref(self.rec), k, self.sg_target, self.mxy, self.mzz, d))
         else:

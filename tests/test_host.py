@@ -144,3 +144,78 @@ def test_synth_generator_deterministic_and_voids():
     cfg2 = dict(cfg, seed=cfg["seed"] + 1)
     x3, _ = synth.make_atoms(cfg2)
     assert not np.array_equal(x1[:100], x3[:100])
+
+
+def _golden_ng_imaging(golden_dir):
+    """The W loop golden case set up as d2bi does for -lattice bcc -k 0,0,1
+    -g 2,0,0 (image extents from the atoms in the standard frame,
+    d2bi.f90:1198-1230; constructor extinction distances, :1265)."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    a0 = lattice_a0(3.1652)
+    box = w["box"][:, 1] - w["box"][:, 0]
+    sa, sn = input_supercell(np.diag(box), 4, a0)
+    U = (C.c_double * 9)()
+    abi.load().d2bi_rotate_to_standard_frame((C.c_double * 3)(0, 0, 1), abi.mat_cm(a0 * np.eye(3)),
+                                             (C.c_double * 3)(2, 0, 0), U)
+    y = w["x"] @ abi.mat_from_cm(U).T
+    im = DynamicalTwoBeamImaging(sa, sn, a0, (0, 0, 1), (2, 0, 0), 200.0, 4, 4, latt="bcc")
+    im.setConventionalUnitCell(a0 * np.eye(3))
+    im.setVoxelSupercellSize(y.max(0) - y.min(0))
+    return w, im
+
+
+def _rot_angle(Ra, Rb):
+    c = (np.trace(Ra.T @ Rb) - 1) / 2
+    return math.acos(max(-1.0, min(1.0, c)))
+
+
+def test_ng_tilt_search_golden_geometry(golden_dir):
+    """setSupercellTilt1a(ng = 1) on the W loop (testD2BI.sh's condition)
+    against the golden .dat header written by the reference with that
+    search.  The tilt about x lands 4 steps of the 2000-point theta grid
+    (1.7e-4 rad) from the header's: the scores near the minimum agree to
+    1e-4 and the header's extinction distances (113.88929, 308.24709 A) show
+    it was made with another version of the W table than the one the
+    reference ships (113.88742, 308.21269), which is enough to move the
+    arg-min.  The tilt about y (dark-field golden section), s_g and the
+    rotated depth are pinned to the header's printed digits."""
+    w, im = _golden_ng_imaging(golden_dir)
+    im.setSupercellTilt(ng=1.0, extinction_distances_filename=os.path.join(golden_dir, "W_V200_T300.dat"))
+    U, R = im.getRotation()
+    RU = R @ U
+    assert np.allclose(RU @ RU.T, np.eye(3), atol=1e-12) and abs(np.linalg.det(RU) - 1) < 1e-12
+    hdr = w["rotation_cm"].reshape(3, 3).T
+    assert np.max(np.abs(RU - hdr)) < 2.5e-4
+    assert abs(RU[0, 2] - hdr[0, 2]) < 1e-6          # tilt about y: -0.019710
+    assert abs(im.getDeviationParameter() - w["sg"]) < 5e-6
+    assert np.allclose(im.d[:2], w["space"][:2], atol=5e-4)
+    assert abs(im.d[2] - w["space"][2]) < 2e-3
+    # U takes [001] to the beam and g = [200] to x (findBestRotationMatrix)
+    assert np.allclose(U @ np.array([0, 0, 1.0]), [0, 0, 1], atol=1e-14)
+    assert np.allclose(U @ np.array([1.0, 0, 0]), [1, 0, 0], atol=1e-14)
+
+
+def test_ng_fractional_is_slerp_of_neighbours(golden_dir):
+    """ng = 1.5 interpolates the tilts found for ng = 1 and ng = 2 by
+    quaternion slerp: it lies on the geodesic, half-way (Lib_Quaternions.f90
+    slerp, :526-564).  An integer ng round-trips R through the quaternion."""
+    xi = os.path.join(golden_dir, "W_V200_T300.dat")
+    Rs = {}
+    for ng in (1.0, 2.0, 1.5):
+        _, im = _golden_ng_imaging(golden_dir)
+        im.setSupercellTilt(ng=ng, extinction_distances_filename=xi)
+        Rs[ng] = im.getRotation()[1]
+    full = _rot_angle(Rs[1.0], Rs[2.0])
+    assert full > 1e-3
+    assert abs(_rot_angle(Rs[1.0], Rs[1.5]) - full / 2) < 1e-9
+    assert abs(_rot_angle(Rs[1.5], Rs[2.0]) - full / 2) < 1e-9
+    for R in Rs.values():
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-13)
+
+
+def test_ng_tilt_errors(golden_dir, tmp_path):
+    _, im = _golden_ng_imaging(golden_dir)
+    with pytest.raises(RuntimeError, match="not found"):
+        im.setSupercellTilt(ng=1.0, extinction_distances_filename=str(tmp_path / "missing.dat"))
+    with pytest.raises(RuntimeError, match="ng must be"):
+        im.setSupercellTilt(ng=0.5, extinction_distances_filename=os.path.join(golden_dir, "W_V200_T300.dat"))

@@ -46,6 +46,11 @@ def test_usage_and_errors(tmp_path):
     assert r.returncode != 0 and "could not find input file" in (r.stdout + r.stderr)
     (tmp_path / "a.lammps").write_text("x")
     r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1", "-ng", "1"], tmp_path)
+    assert r.returncode != 0 and "needs -xifile" in (r.stdout + r.stderr)
+    r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1", "-ng", "1", "-sg", "0.01"],
+            tmp_path)
+    assert r.returncode != 0 and "both n_g and s_g" in (r.stdout + r.stderr)
+    r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1", "-alterg"], tmp_path)
     assert r.returncode != 0 and "not part of this build" in (r.stdout + r.stderr)
     r = run(["-f", "a.lammps", "-a0", "3.1652", "-g", "2,0,0", "-k", "0,0,1", "-xifile", "x.dat",
              "-xi0", "100"], tmp_path)
@@ -77,12 +82,39 @@ def _read_png16(path):
     return px[:, :, 0] * 256 + px[:, :, 1]
 
 
+def d2bi_script_gate(got):
+    """testD2BI.sh's pass test: int(max |(x - target)/(1/256) + 0.5|) == 0
+    over min, max and avg, i.e. x - target in (-1.5/256, 0.5/256)."""
+    acc = 0.0039
+    worst = max(abs((a - b) / acc + 0.5) for a, b in zip(got, (0.000028, 0.230127, 0.081501)))
+    return int(worst) == 0
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_testd2bi_ng_run_through_host(tmp_path, golden_dir):
+    """testD2BI.sh itself: -lattice bcc -k 0,0,1 -g 2,0,0 -xifile <W table>
+    -ng 1 -o "" on the W loop.  The tilt comes from this build's
+    (g,ng) search; the printed minmaxavg must pass the script's gate."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    write_lammps(tmp_path / "w.lammps", w["x"], w["box"])
+    xi = os.path.join(golden_dir, "W_V200_T300.dat")
+    for mode in ("fp64", "fast"):
+        r = run(["-f", "w.lammps", "-a0", "3.1652", "-lattice", "bcc", "-k", "0,0,1", "-g", "2,0,0",
+                 "-xifile", xi, "-ng", "1", "-o", "", "-mode", mode], tmp_path)
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"minmaxavg d2bimg\s+(\S+)\s+(\S+)\s+(\S+)", r.stdout)
+        got = [float(v) for v in m.groups()]
+        print("[testD2BI -ng 1] %s minmaxavg %s" % (mode, got), flush=True)
+        assert d2bi_script_gate(got), got
+
+
 @needs_exe
 @pytest.mark.gpu
 def test_golden_run_through_host(tmp_path, golden_dir):
-    """testD2BI.sh's run, with the golden image's rotation given by -U/-R
-    (the tilt search -ng is not in this build): minmaxavg within 1/256 of
-    0.000028/0.230127/0.081501, .dat within 1e-4 of the golden image."""
+    """testD2BI.sh's run, with the golden image's rotation given by -U/-R:
+    minmaxavg within 1/256 of 0.000028/0.230127/0.081501, .dat within 1e-4
+    of the golden image."""
     w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
     write_lammps(tmp_path / "w.lammps", w["x"], w["box"])
     write_xi(tmp_path / "xi.dat", w["xi"][0], w["xi"][1], (2, 0, 0))
