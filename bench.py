@@ -31,6 +31,11 @@ PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector/matrix peak
 PEAK_TRANS_PER_S = 157.3e12 / 16     # transcendentals: quarter of the non-packed FP32 op rate
 FLOPS_PER_VOXEL = 388.0
 TRANS_PER_VOXEL = 30.0
+# HBM bytes per k_voxel_tile launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes scaled by a 1 GiB copy's counts (profiles/k_voxel_tile_c1_traffic.md);
+# a profiler cannot run inside the timed process, so the figure is recorded
+# per workload
+TRAFFIC_BYTES = {"c1": 2.950e9}
 
 
 def log(*a):
@@ -127,7 +132,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--fp64-steps", type=int, default=3)
-    ap.add_argument("--cpu-cols", type=int, default=24)
+    ap.add_argument("--cpu-cols", type=int, default=384,
+                    help="side of the column window the CPU baseline images (~10 s on 16 threads for c1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", default="c3", help="batched workload ('' to skip)")
     ap.add_argument("--batch-steps", type=int, default=2)
@@ -201,7 +207,8 @@ def main():
     achieved = vox * FLOPS_PER_VOXEL / t_vox / 1e12
     roof = {"bound": "mfma", "pipe": "valu_fp32", "kernel": "k_voxel_tile",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES.get(args.workload),
+            "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
             "ms_per_launch": t_vox * 1e3,
             "transcendental_frac": vox * TRANS_PER_VOXEL / t_vox / PEAK_TRANS_PER_S}
     out = {

@@ -461,40 +461,51 @@ __global__ void k_bin_place(int64_t nr, const int* __restrict__ pt_cell, const i
     cell_ids[cell_start[pt_cell[ii]] + pt_slot[ii]] = (int)ii;
 }
 
-// Pass 4: restore insertion order inside each cell (sort by ii, which is
-// the reference's add order) and gather slot-ordered point data: 1-based
-// id, FP64 offset in cell, FP64 and FP32 (cos, sin) of g.r.
-__global__ void k_bin_finish(int ncell, const int* __restrict__ cell_start, int* __restrict__ cell_ids,
-                             const double* __restrict__ pt_frac, const double* __restrict__ pt_gdotr,
-                             double a0, double a1, double a2,
-                             int* __restrict__ slot_id, double4* __restrict__ slot_frac,
-                             double2* __restrict__ slot_cs64, float4* __restrict__ slot_p32,
-                             float* __restrict__ slot_sn32, int* __restrict__ max_per_cell)
+// Pass 4a: restore insertion order inside each cell (sort by ii, which is
+// the reference's add order) and the largest cell count (one atomic per
+// wave).
+__global__ __launch_bounds__(256) void k_bin_sort(int ncell, const int* __restrict__ cell_start,
+                                                  int* __restrict__ cell_ids, int* __restrict__ max_per_cell)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    int s0 = cell_start[c], s1 = cell_start[c + 1];
-    for (int i = s0 + 1; i < s1; i++) {                 // insertion sort (cells hold a few points)
-        int v = cell_ids[i];
-        int k = i - 1;
-        while (k >= s0 && cell_ids[k] > v) { cell_ids[k + 1] = cell_ids[k]; k--; }
-        cell_ids[k + 1] = v;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = 0;
+    if (c < ncell) {
+        const int s0 = cell_start[c], s1 = cell_start[c + 1];
+        for (int i = s0 + 1; i < s1; i++) {             // insertion sort (cells hold a few points)
+            const int v = cell_ids[i];
+            int k = i - 1;
+            while (k >= s0 && cell_ids[k] > v) { cell_ids[k + 1] = cell_ids[k]; k--; }
+            cell_ids[k + 1] = v;
+        }
+        m = s1 - s0;
     }
-    for (int s = s0; s < s1; s++) {
-        int ii = cell_ids[s];
-        slot_id[s] = ii + 1;
-        slot_frac[s] = make_double4(pt_frac[3 * (int64_t)ii], pt_frac[3 * (int64_t)ii + 1],
-                                    pt_frac[3 * (int64_t)ii + 2], 0.0);
-        double sn, cs;
-        sincos(pt_gdotr[ii], &sn, &cs);
-        slot_cs64[s] = make_double2(cs, sn);
-        // FAST-mode record: offset from the cell origin in A (cuboidal cells)
-        // and cos(g.r), all rounded from FP64; sin(g.r) beside it
-        slot_p32[s] = make_float4((float)(a0 * pt_frac[3 * (int64_t)ii]), (float)(a1 * pt_frac[3 * (int64_t)ii + 1]),
-                                  (float)(a2 * pt_frac[3 * (int64_t)ii + 2]), (float)cs);
-        slot_sn32[s] = (float)sn;
-    }
-    if (s1 - s0 > 0) atomicMax(max_per_cell, s1 - s0);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(max_per_cell, m);
+}
+
+// Pass 4b: slot-ordered point data, one thread per slot: 1-based id, FP64
+// offset in cell, FP64 (cos, sin) of g.r, and the FAST record (offset from
+// the cell origin in A for cuboidal cells, cos g.r; sin g.r beside it), all
+// rounded from FP64.
+__global__ __launch_bounds__(256) void k_bin_gather(int64_t nr, const int* __restrict__ cell_ids,
+                                                    const double* __restrict__ pt_frac,
+                                                    const double* __restrict__ pt_gdotr, double a0, double a1,
+                                                    double a2, int* __restrict__ slot_id,
+                                                    double4* __restrict__ slot_frac, double2* __restrict__ slot_cs64,
+                                                    float4* __restrict__ slot_p32, float* __restrict__ slot_sn32)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nr) return;
+    const int ii = cell_ids[s];
+    slot_id[s] = ii + 1;
+    const double f0 = pt_frac[3 * (int64_t)ii], f1 = pt_frac[3 * (int64_t)ii + 1], f2 = pt_frac[3 * (int64_t)ii + 2];
+    slot_frac[s] = make_double4(f0, f1, f2, 0.0);
+    double sn, cs;
+    sincos(pt_gdotr[ii], &sn, &cs);
+    slot_cs64[s] = make_double2(cs, sn);
+    slot_p32[s] = make_float4((float)(a0 * f0), (float)(a1 * f1), (float)(a2 * f2), (float)cs);
+    slot_sn32[s] = (float)sn;
 }
 
 struct Binned {
@@ -502,6 +513,19 @@ struct Binned {
     int ncell = 0;
     int max_per_cell = 0;
 };
+
+// Pass 4 (sort within cells, then gather by slot).
+static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s)
+{
+    Workspace& w = g.ws;
+    hipLaunchKernelGGL(k_bin_sort, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, w.cell_start.as<int>(),
+                       w.cell_ids.as<int>(), mpc);
+    if (nr > 0)
+        hipLaunchKernelGGL(k_bin_gather, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
+                           w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(), P.cell_a[0],
+                           P.cell_a[1], P.cell_a[2], w.slot_id.as<int>(), w.slot_frac.as<double4>(),
+                           w.slot_cs64.as<double2>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
+}
 
 // Run binning for either a condition (generic = 0) or plain points.
 static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, hipStream_t s)
@@ -559,11 +583,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
                            w.cell_ids.as<int>());
         HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_bin_finish, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell,
-                       w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
-                       w.pt_gdotr.as<double>(), P.cell_a[0], P.cell_a[1], P.cell_a[2], w.slot_id.as<int>(),
-                       w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), w.slot_p32.as<float4>(),
-                       w.slot_sn32.as<float>(), w.counters.as<int>());
+    launch_bin_finish(ncell, nr, P, w.counters.as<int>(), s);
     HIPCHK(hipGetLastError());
     out->nr = nr;
     out->ncell = ncell;
@@ -1912,10 +1932,7 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
     if (c.nr > 0)
         hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((c.nr + 255) / 256)), dim3(256), 0, s, c.nr,
                            w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.cell_start.as<int>(), w.cell_ids.as<int>());
-    hipLaunchKernelGGL(k_bin_finish, dim3((c.ncell + 255) / 256), dim3(256), 0, s, c.ncell, w.cell_start.as<int>(),
-                       w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(), c.BP.cell_a[0],
-                       c.BP.cell_a[1], c.BP.cell_a[2], w.slot_id.as<int>(), w.slot_frac.as<double4>(),
-                       w.slot_cs64.as<double2>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), mpc);
+    launch_bin_finish(c.ncell, c.nr, c.BP, mpc, s);
     HIPCHK(hipGetLastError());
     VoxParams V;
     d2bi_record dummy;
