@@ -127,6 +127,7 @@ struct State {
     bool init = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev[10] = {};
+    bool pinned_busy = false;       // ev[9] marks the last copy out of the pinned staging buffer
     Workspace ws;
 };
 
@@ -138,8 +139,15 @@ static int need_init()
     return D2BI_OK;
 }
 
+// The pinned staging buffer: a caller that leaves an asynchronous copy from
+// it in flight records ev[9] and sets pinned_busy; the next user waits for
+// that copy before writing into (or replacing) the buffer.
 static int pinned(size_t bytes, void** out)
 {
+    if (g.pinned_busy) {
+        HIPCHK(hipEventSynchronize(g.ev[9]));
+        g.pinned_busy = false;
+    }
     if (bytes > g.ws.pinned_cap) {
         if (g.ws.pinned) (void)hipHostFree(g.ws.pinned);
         g.ws.pinned = nullptr;
@@ -364,13 +372,21 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __rest
                 // the magnitudes involved; eps is 1e-10 of their bound
                 const double eps =
                     1.0e-10 * (1.0 + P.mabs * (fabs(x3[0]) + fabs(x3[1]) + fabs(x3[2])) + P.oabs);
-                const double E[3] = {(double)P.Ex, (double)P.Ey, (double)P.Ez};
+                // c in [0, E) per axis is |c - E/2| < E/2: certainly outside
+                // when |c - E/2| >= E/2 + eps, certainly inside when <= E/2 - eps
+                const double h[3] = {0.5 * P.Ex, 0.5 * P.Ey, 0.5 * P.Ez};
+                double ch[3], hout[3], hin[3];
+                for (int i = 0; i < 3; i++) {
+                    ch[i] = cb[i] - h[i];
+                    hout[i] = h[i] + eps;
+                    hin[i] = h[i] - eps;
+                }
                 for (int nn = 0; nn < P.nuvw; nn++) {
                     bool out = false, sure = true;
                     for (int i = 0; i < 3; i++) {
-                        const double c = cb[i] + P.off[3 * nn + i];
-                        out = out || c <= -eps || c >= E[i] + eps;
-                        sure = sure && (c <= -eps || c >= E[i] + eps || (c >= eps && c <= E[i] - eps));
+                        const double u = fabs(ch[i] + P.off[3 * nn + i]);
+                        out = out || u >= hout[i];
+                        sure = sure && u <= hin[i];
                     }
                     if (out) continue;                       // outside on some axis for certain
                     if (sure) { m |= 1u << nn; continue; }   // inside on every axis for certain
@@ -430,9 +446,11 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                 double yy[3], cyy[3];
                 bin_point(P, x3, nn, yy, cyy);
                 double fl[3] = {floor(cyy[0]), floor(cyy[1]), floor(cyy[2])};
-                int cx = cell_of(fl[0], P.Ex);
-                int cy = cell_of(fl[1], P.Ey);
-                int cz = cell_of(fl[2], P.Ez);
+                // a replica point passed pointInSupercell, so 0 <= floor < E
+                // and mod(floor + 16E, E) is the floor itself
+                const int cx = P.generic ? cell_of(fl[0], P.Ex) : (int)fl[0];
+                const int cy = P.generic ? cell_of(fl[1], P.Ey) : (int)fl[1];
+                const int cz = P.generic ? cell_of(fl[2], P.Ez) : (int)fl[2];
                 int cell = cx + P.Ex * (cy + P.Ey * cz);
                 pt_cell[ii] = cell;
                 pt_frac[3 * (int64_t)ii + 0] = cyy[0] - fl[0];
@@ -463,21 +481,68 @@ __global__ void k_bin_place(int64_t nr, const int* __restrict__ pt_cell, const i
 
 // Pass 4a: restore insertion order inside each cell (sort by ii, which is
 // the reference's add order) and the largest cell count (one atomic per
-// wave).
+// wave).  A block takes 256 consecutive cells: their points are one
+// contiguous range, loaded into LDS with coalesced reads when it fits,
+// sorted there segment by segment (one thread per cell: cells of up to
+// SORT_K points are ranked in registers, fully unrolled; larger ones are
+// insertion-sorted), and written back coalesced.  A range that does not fit
+// is sorted in place in memory.
+static constexpr int SORT_K = 16;
+static constexpr int SORT_LDS = 4096;                // ints of LDS per block
+
+// n >= 1.  Every read stays inside the cell: the index is clamped before
+// the select, so a load the compiler issues unconditionally is still in
+// bounds.
+template <typename Ids>
+__device__ __forceinline__ void sort_cell(Ids ids, int s0, int n)
+{
+    if (n <= SORT_K) {
+        int v[SORT_K];
+#pragma unroll
+        for (int k = 0; k < SORT_K; k++) {
+            const int x = ids[s0 + min(k, n - 1)];
+            v[k] = k < n ? x : 0x7fffffff;
+        }
+#pragma unroll
+        for (int i = 0; i < SORT_K; i++) {
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < SORT_K; j++) r += v[j] < v[i] ? 1 : 0;
+            if (i < n) ids[s0 + r] = v[i];
+        }
+    } else {
+        for (int i = s0 + 1; i < s0 + n; i++) {
+            const int v = ids[i];
+            int k = i - 1;
+            while (k >= s0 && ids[k] > v) { ids[k + 1] = ids[k]; k--; }
+            ids[k + 1] = v;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_bin_sort(int ncell, const int* __restrict__ cell_start,
                                                   int* __restrict__ cell_ids, int* __restrict__ max_per_cell)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    int m = 0;
+    __shared__ int sid[SORT_LDS];
+    const int c0 = blockIdx.x * blockDim.x;
+    const int c = c0 + threadIdx.x;
+    const int cl = min(c0 + (int)blockDim.x, ncell);
+    const int R0 = cell_start[c0], R1 = cell_start[cl];
+    const bool in_lds = R1 - R0 <= SORT_LDS;
+    int m = 0, s0 = 0, n = 0;
     if (c < ncell) {
-        const int s0 = cell_start[c], s1 = cell_start[c + 1];
-        for (int i = s0 + 1; i < s1; i++) {             // insertion sort (cells hold a few points)
-            const int v = cell_ids[i];
-            int k = i - 1;
-            while (k >= s0 && cell_ids[k] > v) { cell_ids[k + 1] = cell_ids[k]; k--; }
-            cell_ids[k + 1] = v;
-        }
-        m = s1 - s0;
+        s0 = cell_start[c];
+        n = cell_start[c + 1] - s0;
+        m = n;
+    }
+    if (in_lds) {
+        for (int k = R0 + (int)threadIdx.x; k < R1; k += blockDim.x) sid[k - R0] = cell_ids[k];
+        __syncthreads();
+        if (n > 1) sort_cell(sid, s0 - R0, n);
+        __syncthreads();
+        for (int k = R0 + (int)threadIdx.x; k < R1; k += blockDim.x) cell_ids[k] = sid[k - R0];
+    } else if (n > 1) {
+        sort_cell(cell_ids, s0, n);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
@@ -766,8 +831,9 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
     HIPCHK(hipMemcpyAsync(w.ax_zi.p, hz, sizeof(double) * nc, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.ax_start.p, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.ax_ql.p, hq, sizeof(QL) * nc, hipMemcpyHostToDevice, s));
-    // the pinned buffer is reused by later calls: wait until the copies are done
-    HIPCHK(hipStreamSynchronize(s));
+    // the pinned buffer is reused by later calls: they wait for this event
+    HIPCHK(hipEventRecord(g.ev[9], s));
+    g.pinned_busy = true;
     return D2BI_OK;
 }
 
@@ -1348,28 +1414,41 @@ __global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const v
     float p0r[NP], p0i[NP], pgr[NP], pgi[NP];
 #pragma unroll
     for (int p = 0; p < NP; p++) { p0r[p] = 1.0f; p0i[p] = 0.0f; pgr[p] = 0.0f; pgi[p] = 0.0f; }
-    // field: float4 {x, rho} with the density field, else float2 x (rho = 1)
-    auto coef = [&](int k, float& c0, float& cgr, float& cgi) {
+    // field: float4 {x, rho} with the density field, else float2 x (rho = 1);
+    // sample Mz is sample 0.  Samples are loaded one double-step ahead.
+    auto load = [&](int k) -> float4 {
         const int64_t o = col + plane * (k == nz ? 0 : k);
+        if (DENS) return static_cast<const float4*>(field)[o];
+        const float2 f = static_cast<const float2*>(field)[o];
+        return make_float4(f.x, f.y, 1.0f, 0.0f);
+    };
+    auto coef = [&](const float4& f, float& c0, float& cgr, float& cgi) {
         if (DENS) {
-            const float4 f = static_cast<const float4*>(field)[o];
             c0 = P.b0 * f.z;
             const float bgr = P.bg * f.z;
             cgr = bgr * f.x;
             cgi = bgr * f.y;
         } else {
-            const float2 f = static_cast<const float2*>(field)[o];
             c0 = P.b0;
             cgr = P.bg * f.x;
             cgi = P.bg * f.y;
         }
     };
     float c0a = 0, cgra = 0, cgia = 0;
-    if (nz >= 2) coef(0, c0a, cgra, cgia);
+    float4 fb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), fc = fb;
+    if (nz >= 2) {
+        coef(load(0), c0a, cgra, cgia);
+        fb = load(1);
+        fc = load(2);
+    }
     for (int ii = 0; ii <= nz - 2; ii += 2) {
         float c0b, cgrb, cgib, c0c, cgrc, cgic;
-        coef(ii + 1, c0b, cgrb, cgib);
-        coef(ii + 2, c0c, cgrc, cgic);
+        coef(fb, c0b, cgrb, cgib);
+        coef(fc, c0c, cgrc, cgic);
+        if (ii + 4 <= nz) {
+            fb = load(ii + 3);
+            fc = load(ii + 4);
+        }
 #pragma unroll
         for (int p = 0; p < NP; p++) {
             const float a = P.aa[p];
@@ -1805,16 +1884,23 @@ static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, d
     Binned b;
     CHK(run_binning(d_x, n, BP, &b, s));
     CHK(T.record(1));
-    int64_t ng = 0;
-    CHK(run_voxel(r, &plan, tab, ax, mode, false, st ? &ng : nullptr, s, &T));
+    CHK(run_voxel(r, &plan, tab, ax, mode, false, nullptr, s, &T));
     CHK(run_integrate(&plan, r->use_density != 0, mode, s, &T));
     CHK(T.record(6));
+    // counters (largest cell count at 0, FAST guard re-tests at 8) come back
+    // with the final synchronisation
+    int64_t* hc = nullptr;
+    if (st) {
+        void* hp;
+        CHK(pinned(16, &hp));
+        hc = (int64_t*)hp;
+        HIPCHK(hipMemcpyAsync(hc, g.ws.counters.p, 16, hipMemcpyDeviceToHost, s));
+    }
     HIPCHK(hipStreamSynchronize(s));
     if (st) {
-        int mpc = 0;
-        HIPCHK(hipMemcpy(&mpc, g.ws.counters.p, sizeof(int), hipMemcpyDeviceToHost));
+        const int mpc = (int)(hc[0] & 0xffffffff);
         st->n_points += b.nr;
-        st->n_guard += ng;
+        st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
         st->max_per_cell = std::max(st->max_per_cell, mpc);
         st->n_cells = b.ncell;
         st->ms_bin += T.ms(0, 1);
