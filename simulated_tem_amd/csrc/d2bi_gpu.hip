@@ -97,6 +97,8 @@ struct Workspace {
     Buf cell_cnt, cell_start, cell_ids;
     Buf slot_id, slot_frac, slot_cs64, slot_p32, slot_sn32;
     Buf scan_tmp;
+    Buf blk_max;                // per-block largest cell count (link-cell sort)
+    Buf guard_part;             // FAST guard re-test counters, one per 128-byte line
     Buf counters;               // misc device counters (guard count, max per cell)
     // voxel axis tables
     Buf ax_cell, ax_zi, ax_start, ax_ql;
@@ -112,7 +114,7 @@ struct Workspace {
     {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
-                      &slot_p32, &slot_sn32, &scan_tmp, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
+                      &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_k};
         for (Buf* b : all) b->release();
@@ -457,7 +459,23 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                 pt_frac[3 * (int64_t)ii + 1] = cyy[1] - fl[1];
                 pt_frac[3 * (int64_t)ii + 2] = cyy[2] - fl[2];
                 pt_gdotr[ii] = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
-                pt_slot[ii] = atomicAdd(&cell_cnt[cell], 1);
+                // arrival slot: lanes of this wave landing in one cell share
+                // one atomic (the lowest such lane adds the group's size);
+                // the grouping is ALU-only, so each lane waits for one atomic
+                // return.  Order inside a cell is restored by ii later.
+                unsigned long long rem = __ballot(1), grp = 0;
+                int leader = lane;
+                while (rem) {
+                    const int l = __ffsll((long long)rem) - 1;
+                    const int lc = __shfl(cell, l, 64);
+                    const unsigned long long gm = __ballot(cell == lc);
+                    if (cell == lc) { leader = l; grp = gm; }
+                    rem &= ~gm;
+                }
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&cell_cnt[cell], (int)__popcll(grp));
+                base = __shfl(base, leader, 64);
+                pt_slot[ii] = base + (int)__popcll(grp & lt);
             }
         }
         __syncthreads();
@@ -521,8 +539,9 @@ __device__ __forceinline__ void sort_cell(Ids ids, int s0, int n)
 }
 
 __global__ __launch_bounds__(256) void k_bin_sort(int ncell, const int* __restrict__ cell_start,
-                                                  int* __restrict__ cell_ids, int* __restrict__ max_per_cell)
+                                                  int* __restrict__ cell_ids, int* __restrict__ blk_max)
 {
+    __shared__ int wmax[4];
     __shared__ int sid[SORT_LDS];
     const int c0 = blockIdx.x * blockDim.x;
     const int c = c0 + threadIdx.x;
@@ -544,9 +563,39 @@ __global__ __launch_bounds__(256) void k_bin_sort(int ncell, const int* __restri
     } else if (n > 1) {
         sort_cell(cell_ids, s0, n);
     }
+    // the block's largest cell count, stored (no atomics on one address)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(max_per_cell, m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_max[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+// Largest of n per-block counts into *out (one block).
+__global__ __launch_bounds__(256) void k_max_i32(const int* __restrict__ v, int n, int* __restrict__ out)
+{
+    __shared__ int wmax[4];
+    int m = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = max(m, v[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+// Sum of the FAST guard counters into *out (one block).
+__global__ __launch_bounds__(256) void k_sum_guard(const unsigned long long* __restrict__ part, int n, int stride,
+                                                   unsigned long long* __restrict__ out)
+{
+    __shared__ unsigned long long ws[4];
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[(int64_t)i * stride];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
 // Pass 4b: slot-ordered point data, one thread per slot: 1-based id, FP64
@@ -583,8 +632,10 @@ struct Binned {
 static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s)
 {
     Workspace& w = g.ws;
-    hipLaunchKernelGGL(k_bin_sort, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, w.cell_start.as<int>(),
-                       w.cell_ids.as<int>(), mpc);
+    const int nb = (ncell + 255) / 256;         // blk_max holds >= nb ints (sized with cell_start)
+    hipLaunchKernelGGL(k_bin_sort, dim3(nb), dim3(256), 0, s, ncell, w.cell_start.as<int>(), w.cell_ids.as<int>(),
+                       w.blk_max.as<int>());
+    hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(256), 0, s, w.blk_max.as<int>(), nb, mpc);
     if (nr > 0)
         hipLaunchKernelGGL(k_bin_gather, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
                            w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(), P.cell_a[0],
@@ -606,6 +657,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     CHK(w.blk_off.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
     CHK(w.cell_cnt.ensure(sizeof(int) * (ncell + 1)));
     CHK(w.cell_start.ensure(sizeof(int) * (ncell + 1)));
+    CHK(w.blk_max.ensure(sizeof(int) * ((ncell + 255) / 256 + 1)));
     CHK(w.counters.ensure(64));
     HIPCHK(hipMemsetAsync(w.cell_cnt.p, 0, sizeof(int) * (ncell + 1), s));
     HIPCHK(hipMemsetAsync(w.counters.p, 0, 64, s));
@@ -996,7 +1048,9 @@ __global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const doubl
 
 // ------------------------------------------------------------------ (b) voxelise, FAST
 
-static constexpr int TILE_CAP = 160;                 // staged candidates per pass (LDS ~5 KB/wave)
+static constexpr int TILE_CAP = 160;
+static constexpr int GUARD_SLOTS = 1024;             // guard counters (power of two) ...
+static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart                 // staged candidates per pass (LDS ~5 KB/wave)
 static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
 
 struct FastParams {
@@ -1385,7 +1439,8 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             out_r2m[o] = r2m;
         }
     }
-    if (act && nguard) atomicAdd(guard_count, (unsigned long long)nguard);
+    // spread over GUARD_SLOTS lines: many waves adding into one address serialise
+    if (act && nguard) atomicAdd(guard_count + GUARD_STRIDE * (blockIdx.x & (GUARD_SLOTS - 1)), (unsigned long long)nguard);
 }
 
 // ------------------------------------------------------------------ (c) integrate, FAST
@@ -1759,12 +1814,14 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     CHK(fast_params(p, tab, V, &F));
     CHK(w.counters.ensure(64));
     unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
-    HIPCHK(hipMemsetAsync(gc, 0, 8, s));
+    const size_t gp_bytes = sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE;
+    CHK(w.guard_part.ensure(gp_bytes));
+    unsigned long long* gp = w.guard_part.as<unsigned long long>();
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, w.cell_start.as<int>(),
                            w.slot_frac.as<double4>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>(),
                            w.f32_field.as<float4>(), dcnt,
-                           (float*)dr2, gc);
+                           (float*)dr2, gp);
     };
     const bool dens = r->use_density != 0;
     auto run_variant = [&](int v) {
@@ -1782,11 +1839,12 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         return ms;
     });
     if (variant < 0) return variant;
-    HIPCHK(hipMemsetAsync(gc, 0, 8, s));     // tuning launches counted too
+    HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches counted too
     if (T) CHK(T->record(2));
     run_variant(variant);
     HIPCHK(hipGetLastError());
     if (T) CHK(T->record(3));
+    hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
     if (n_guard) {
         unsigned long long v = 0;
         HIPCHK(hipMemcpyAsync(&v, gc, 8, hipMemcpyDeviceToHost, s));
@@ -2008,7 +2066,9 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
     unsigned long long* gc = B.guard.as<unsigned long long>() + idx;
     fill_i32(w.cell_cnt.as<int>(), c.ncell + 1, 0, s);
     fill_i32(mpc, 1, 0, s);
+    unsigned long long* gp = w.guard_part.as<unsigned long long>();
     fill_i32((int*)gc, 2, 0, s);
+    if (mode == D2BI_MODE_FAST) fill_i32((int*)gp, 2 * GUARD_SLOTS * GUARD_STRIDE, 0, s);
     if (c.n > 0 && c.nr > 0)
         hipLaunchKernelGGL(k_bin_scatter, dim3(c.nblk), dim3(BIN_BLOCK), 0, s, c.d_x, c.BP,
                            B.mask.as<uint32_t>() + c.o_mask, B.blkoff.as<int>() + c.o_blk, c.nblk,
@@ -2037,8 +2097,9 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
     CHK(fast_params(p, c.tab, V, &F));
     launch_tile<false>(F, c.variant, c.dens, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
-                       gc, s);
+                       gp, s);
     HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
     return launch_integrate_fast(p, c.dens, w.f32_field.p, d_img, s);
 }
 
@@ -2104,6 +2165,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         CHK(B.img.ensure(sizeof(double) * o_img));
         CHK(B.aa.ensure(sizeof(double) * std::max<int64_t>(o_aa, 1)));
         CHK(B.guard.ensure(sizeof(unsigned long long) * nRec));
+        CHK(g.ws.guard_part.ensure(sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE));
         CHK(B.mpc.ensure(sizeof(int) * nRec));
         CHK(g.ws.scan_tmp.ensure(sizeof(int) * max_scan));
         // tables and precession coefficients stay resident in the batch buffers
@@ -2163,6 +2225,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         CHK(w.slot_sn32.ensure(sizeof(float) * max_nr));
         CHK(w.cell_cnt.ensure(sizeof(int) * max_cell));
         CHK(w.cell_start.ensure(sizeof(int) * max_cell));
+        CHK(w.blk_max.ensure(sizeof(int) * (max_cell / 256 + 2)));
         if (mode == D2BI_MODE_FP64) {
             CHK(w.f64_re.ensure(sizeof(double) * max_vox));
             CHK(w.f64_im.ensure(sizeof(double) * max_vox));
@@ -2316,6 +2379,25 @@ int d2bi_gpu_image_dev(const d2bi_record* rec, d2bi_atoms atoms, double precAngl
     d2bi_plan plan;
     CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
     HIPCHK(hipMemcpy(img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToHost));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_image_dev_out(const d2bi_record* rec, d2bi_atoms atoms, double precAngle, int32_t nPrec,
+                           int32_t mode, double* d_img, d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (!rec || !atoms || !d_img) return seterr(D2BI_ERR_ARG, "null argument");
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, d_img) != hipSuccess) {
+        (void)hipGetLastError();                 // a failed query must not linger as the last error
+        return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    }
+    if (attr.type != hipMemoryTypeDevice) return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    d2bi_plan plan;
+    CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
+    HIPCHK(hipMemcpyAsync(d_img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
     return D2BI_OK;
 }
 

@@ -70,6 +70,52 @@ class Atoms:
             self.h = None
 
 
+class DeviceBuffer:
+    """A float64 array in HBM, allocated with the HIP runtime the library
+    itself links (a second HIP runtime in the process, such as the one
+    bundled with a torch wheel, could not open the device).  Used to keep an
+    image in HBM (computeD2BI_img(..., out=buffer))."""
+
+    def __init__(self, shape):
+        _ensure()
+        self.shape = tuple(int(v) for v in shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        self._hip = _hip_runtime()
+        p = C.c_void_p()
+        rc = self._hip.hipMalloc(C.byref(p), C.c_size_t(max(self.nbytes, 8)))
+        if rc != 0:
+            raise MemoryError("hipMalloc failed (%d)" % rc)
+        self.ptr = p.value
+
+    def numpy(self):
+        """Copy to a host array."""
+        out = np.empty(self.shape)
+        rc = self._hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), C.c_size_t(self.nbytes), 2)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed (%d)" % rc)
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _device is not None:
+            self._hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+_hip = None
+
+
+def _hip_runtime():
+    """libamdhip64 as already loaded by libd2bi_gpu.so."""
+    global _hip
+    if _hip is None:
+        abi.load()
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _hip
+
+
 def input_supercell(a_super, M, a0):
     """Voxelised input supercell of d2bi (d2bi.f90:986-992): N_i =
     max(3, floor(M * |A_i| / a0)), cell vectors A_i / N_i."""
@@ -177,14 +223,24 @@ class DynamicalTwoBeamImaging:
 
     # --- the hot path
     def computeD2BI_img(self, x, precessionAngle=0.0, nPrecessionAngle=1, mode="fast",
-                        stats=None):
-        """Image of one condition, (My, Mx) array.  x: (nAtoms,3) array or Atoms."""
+                        stats=None, out=None):
+        """Image of one condition, (My, Mx) array.  x: (nAtoms,3) array or Atoms.
+        out: optional DeviceBuffer of shape (My, Mx) that receives the image
+        in HBM (x must then be Atoms); it is returned."""
         _ensure()
         L = abi.load()
         p = self.plan(1)
-        img = np.zeros((p.My, p.Mx))
         st = stats if stats is not None else D2biStats()
         m = MODES[mode]
+        if out is not None:
+            if not isinstance(x, Atoms):
+                raise TypeError("a device output needs device-resident Atoms")
+            if not isinstance(out, DeviceBuffer) or out.shape != (p.My, p.Mx):
+                raise ValueError("out must be a DeviceBuffer of shape (%d, %d)" % (p.My, p.Mx))
+            check(L.d2bi_gpu_image_dev_out(C.byref(self.rec), x.h, float(precessionAngle),
+                                           int(nPrecessionAngle), m, C.c_void_p(out.ptr), C.byref(st)))
+            return out
+        img = np.zeros((p.My, p.Mx))
         if isinstance(x, Atoms):
             check(L.d2bi_gpu_image_dev(C.byref(self.rec), x.h, float(precessionAngle),
                                        int(nPrecessionAngle), m, ptr(img), C.byref(st)))

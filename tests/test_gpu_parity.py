@@ -200,6 +200,32 @@ def test_empty_and_single_atom():
 
 
 @pytest.mark.parametrize("mode", ["fast", "fp64"])
+def test_image_device_output(mode):
+    """d2bi_gpu_image_dev_out leaves the image in HBM (a DeviceBuffer):
+    bit-identical to the host-output entry point, density on and off; a
+    host buffer or a wrong shape is refused."""
+    import ctypes as C
+    from simulated_tem_amd import abi
+    for density in (False, True):
+        x, im = _cond("c0", cells=(10, 10, 10), sg=0.01, density=density)
+        at = imaging.Atoms(x)
+        p = im.plan(len(x))
+        host = im.computeD2BI_img(at, 0.005, 3, mode=mode)
+        out = imaging.DeviceBuffer((p.My, p.Mx))
+        got = im.computeD2BI_img(at, 0.005, 3, mode=mode, out=out)
+        assert got is out
+        assert np.array_equal(out.numpy(), host)
+    with pytest.raises(ValueError):
+        im.computeD2BI_img(at, 0.0, 1, out=imaging.DeviceBuffer((p.My, p.Mx + 1)))
+    hostbuf = np.zeros((p.My, p.Mx))
+    rc = abi.load().d2bi_gpu_image_dev_out(C.byref(im.rec), at.h, 0.0, 1, 1,
+                                           C.c_void_p(hostbuf.ctypes.data), None)
+    assert rc == -1
+    # the library still works after the refused call
+    assert np.array_equal(im.computeD2BI_img(at, 0.005, 3, mode=mode), host)
+
+
+@pytest.mark.parametrize("mode", ["fast", "fp64"])
 def test_batch_graph_replays_equal_eager(mode):
     """d2bi_gpu_image_batch captures the whole batch into one HIP graph on the
     first call and replays it afterwards.  Every replay, with eager work and
