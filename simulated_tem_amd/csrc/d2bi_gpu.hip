@@ -1456,6 +1456,9 @@ struct FastIntParams {
 // precession changes s_g only).  FP32 classic RK4 with h = 2 dz; the
 // per-step renormalisation of the reference is a positive scalar on a
 // linear map, so it is applied once at the end: I = |phi_g|^2/|phi|^2.
+// The mean-potential term c0 = b0 rho stays in, although in the exact ODE it
+// is a global phase: RK4's truncation error depends on the whole matrix, and
+// over 800 double steps leaving c0 out moves |phi_g|^2 by ~2e-4.
 template <int NP, bool DENS>
 __global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const void* __restrict__ field,
                                                         double* __restrict__ img, int accumulate,

@@ -300,6 +300,40 @@ def test_golden_w_loop_image(golden_dir):
         assert abs(got - want) <= 0.0039
 
 
+def test_c2_full_size_density_parity():
+    """c2 at full size: 640x640x1600 voxels, 20.5e6 atoms, g = 200, density
+    field on, crystal coordinates up to ~1.4e3 A (g.r up to ~5e3 rad).  All
+    three s_g conditions on the GPU in FP64 and FAST: FAST vs FP64 over the
+    whole image; FP64 and FAST against oracle windows (whole crystal binned)
+    for s_g = -0.02."""
+    print("\n[c2 full] generating 20.5M atoms", flush=True)
+    cfg = dict(synth.CONFIGS["c2"])
+    x, box = synth.make_atoms(cfg)
+    x, conds = synth.conditions(cfg, x, box)
+    assert len(conds) == 3 and conds[0][0].rec.use_density == 1
+    at = imaging.Atoms(x)
+    for k, (im, pa, npr) in enumerate(conds):
+        g64 = im.computeD2BI_img(at, pa, npr, mode="fp64")
+        gf = im.computeD2BI_img(at, pa, npr, mode="fast")
+        assert g64.shape == (640, 640)
+        e = np.abs(gf - g64)
+        print("[c2 full] s_g %+.3f fast vs fp64: max %.3e mean %.3e" % (im.getDeviationParameter(), e.max(),
+                                                                        e.mean()), flush=True)
+        assert e.max() <= FAST_TOL
+        assert np.all(g64 >= 0) and np.all(g64 <= 1)
+        if k == 0:
+            print("[c2 full] oracle binning", flush=True)
+            oc = orc.Condition(im.rec, x, pa, npr)
+            for (ix0, iy0) in [(0, 0), (316, 320), (632, 630)]:
+                ref = oc.image(window=(ix0, ix0 + 8, iy0, iy0 + 10), nthreads=NTH)
+                e64 = np.max(np.abs(g64[iy0:iy0 + 10, ix0:ix0 + 8] - ref))
+                ef = np.max(np.abs(gf[iy0:iy0 + 10, ix0:ix0 + 8] - ref))
+                print("[c2 full] window (%d,%d) fp64 %.3e fast %.3e" % (ix0, iy0, e64, ef), flush=True)
+                assert e64 <= FP64_TOL
+                assert ef <= FAST_TOL
+    del at
+
+
 @pytest.fixture(scope="module")
 def c1_full():
     print("\n[c1 full] generating 8M atoms", flush=True)
