@@ -212,7 +212,9 @@ int d2bi_gpu_image_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int32
 
 /* Voxel field for tests: per voxel (x fastest, then y, then z; Mz planes)
  * the normalised phase factor (re,im), density rho, neighbour count and the
- * minimum squared range.  Any output pointer may be null. */
+ * minimum squared range.  Any output pointer may be null.  In FAST mode with
+ * count and r2min both null the field comes from the imaging kernel itself
+ * (otherwise from its diagnostic form, which also counts neighbours). */
 int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms,
                    int32_t mode, double* phase_re_im, double* rho,
                    int32_t* count, double* r2min);

@@ -1048,9 +1048,9 @@ __global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const doubl
 
 // ------------------------------------------------------------------ (b) voxelise, FAST
 
-static constexpr int TILE_CAP = 160;
+static constexpr int TILE_CAP = 160;                 // staged candidates per pass (lane per voxel)
 static constexpr int GUARD_SLOTS = 1024;             // guard counters (power of two) ...
-static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart                 // staged candidates per pass (LDS ~5 KB/wave)
+static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart
 static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
 
 struct FastParams {
@@ -1064,6 +1064,7 @@ struct FastParams {
     float m2_min;               // (1e-16)^2 / w_scale^2
     float inv_k2;               // 1 / k2
     float cellf[3];             // link-cell sides in FP32
+    int thin_z;                 // link cells thinner than 3 sigma along z
 };
 
 // One wave per 4x4x4 voxel tile.  The tile's candidates are the points of
@@ -1072,8 +1073,8 @@ struct FastParams {
 // mod(u, E) displaced by u - mod(u, E) cells, which is exactly the periodic
 // image neighbourList1b sees through its mod() (Lib_LinkCell3D.f90:1567-1575).
 // Cells, then points, farther than 3 sigma from the tile's voxel box are
-// dropped; the rest are compacted into LDS (ballot + popcount) and every lane
-// (voxel) tests them all.
+// dropped; the rest are compacted into LDS (ballot + popcount) and every
+// voxel of the tile tests them all.
 //
 // Staging.  The kept non-empty cells are listed once per tile with their
 // origin in the tile frame and the offset from candidate index to point
@@ -1092,13 +1093,36 @@ struct FastParams {
 // the guard band the pair is re-tested in FP64 with the expression of
 // neighbourList1b (and the neighbour offset must be -1, 0 or +1, as in its
 // 27-cell loop), so the set of summed atoms is exactly the FP64 set.
-// S = sum w (cos, sin) is accumulated in blocks of 16 candidates whose sums
-// are merged with Kahan compensation.  cos/sin of g.r were evaluated in FP64.
+// cos/sin of g.r were evaluated in FP64.
 //
-// SUB = true splits the wave into eight 2x2x2 voxel groups; each group culls
-// the staged list again against its own voxel box and walks its own index
-// list (about 54 instead of 91 candidates per voxel for bcc at M = 4).
-template <bool DENSITY, bool DIAG, bool SUB>
+// ROW = false: lane = voxel; S = sum w (cos, sin) is accumulated in blocks of
+// 32 candidates whose sums are merged with Kahan compensation.
+//
+// ROW = true: the wave is 16 voxel rows along x (one per (y, z) of the tile)
+// times 4 candidate quarters; lane (row, s) walks candidates s, s+4, s+8, ...
+// for the four voxels of its row.  The Gaussian factorises along x:
+//     2^(e_i - e_rc) = X_i * 2^t,   X_i = 2^(k2 (p_x - q_x,i)^2),
+//     t = k2 ((p_y - q_y)^2 + (p_z - q_z)^2) - e_rc   (e_rc = k2 (3 sigma)^2),
+// and the four x positions q_x,i are the tile's, common to every row, so
+// X_0..X_3 are evaluated once per candidate when it is staged and a lane
+// spends one v_exp_f32 per candidate on four voxels.  Membership e_i >= e_rc
+// is u_i = X_i 2^t - 1 >= 0; a block in which some |u_i| lies within the
+// guard band is walked again and the pairs of the band are re-tested in FP64,
+// correcting the sums where the FP64 test disagrees.  The quarters' sums are
+// added in a fixed order through LDS at the end, after which lane (row, s)
+// holds voxel s of its row: the same lane-to-voxel map as ROW = false.
+//
+// Thin cells.  Link cells are at least 3 sigma wide along x and y (Ex <=
+// Dx/deltaM + 2 with deltaM >= 3 sigma / min(a_x, a_y)), so along those axes
+// every point within 3 sigma of a voxel lies in a neighbouring cell and the
+// range test alone decides membership (the guard band covers rounding of the
+// cell sides).  Along z the cell height Mz a_z / Ez can be smaller (a_z <
+// min(a_x, a_y) with deltaM dividing Mz, or Ez = 3 for thin foils); then a
+// point two cells away can lie within 3 sigma, and neighbourList1b does not
+// see it.  ROW = 2 applies that rule: a candidate whose cell is not within one
+// cell of the row's along z gets weight 0 for the row.  The diagnostic kernel
+// tests the z offset of every pair when F.thin_z is set.
+template <bool DENSITY, bool DIAG, int ROW>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
                                                    const double4* __restrict__ frac,
                                                    const float4* __restrict__ p32,
@@ -1108,6 +1132,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                                                    float* __restrict__ out_r2m,
                                                    unsigned long long* __restrict__ guard_count)
 {
+    static_assert(!(DIAG && ROW), "the diagnostic kernel tests every pair itself");
     // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
     // give each XCD a contiguous range of tiles so neighbouring tiles, which
     // read the same link cells, share one L2.
@@ -1119,11 +1144,9 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const int lane = threadIdx.x;
     const int x0 = tx * TILE, y0 = ty * TILE, z0 = tz * TILE;
     const int x1 = min(P.Mx, x0 + TILE) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TILE) - 1;
-    const int pos = lane & 7, grp = lane >> 3;          // SUB: position in group, group
-    const int ix = SUB ? x0 + (pos & 1) + 2 * (grp & 1) : x0 + (lane & 3);
-    const int iy = SUB ? y0 + ((pos >> 1) & 1) + 2 * ((grp >> 1) & 1) : y0 + ((lane >> 2) & 3);
-    const int iz = SUB ? z0 + (pos >> 2) + 2 * (grp >> 2) : z0 + (lane >> 4);
-    const bool act = ix <= x1 && iy <= y1 && iz <= z1;
+    const int ix = x0 + (lane & 3), iy = y0 + ((lane >> 2) & 3), iz = z0 + (lane >> 4);
+    const bool row_act = iy <= y1 && iz <= z1;
+    const bool act = ix <= x1 && row_act;
     const int jx = min(ix, x1), jy = min(iy, y1), jz = min(iz, z1);
 
     // origin cell of the tile; the voxel's FP32 position relative to it and
@@ -1145,21 +1168,33 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const float qcx = qx - ccx, qcy = qy - ccy, qcz = qz - ccz;
     const float k2 = F.k2;
     const float kq2 = k2 * fmaf(qcx, qcx, fmaf(qcy, qcy, qcz * qcz));
+    // ROW: the tile's four x positions (clamped at the grid's edge), the
+    // lane's candidate quarter and the y-z part of its exponent
+    float qxt[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) qxt[i] = P.ax.ql[0][min(x0 + i, x1)].q - ccx;
+    const int squart = lane & 3;
+    const float kyz = fmaf(k2, fmaf(qcy, qcy, qcz * qcz), -F.e_rc);
 
     // unwrapped cell range (cells are monotonic, so ux0 + dx lies in [-1, E])
     const int ux0 = c0x - 1, uy0 = c0y - 1, uz0 = c0z - 1;
     const int ncx = ex1.lc + 3, ncy = ey1.lc + 3, ncz = ez1.lc + 3;
     const int ncell = ncx * ncy * ncz;                 // <= TILE_MAXC (checked on the host)
 
+    // staged candidates: ROW = false, (-2 k2 p, k2 |p|^2) tile-centred, then
+    // (cos, sin) of g.r, slot, cell offsets; ROW = true, (X_0..X_3), then
+    // (-2 k2 p_y, -2 k2 p_z, k2 (p_y^2 + p_z^2), cos), then (sin, slot, cell
+    // offsets, 0), and four zero records after the last candidate of a pass
+    constexpr int REC = ROW ? 3 : 2;
+    constexpr int CAP = ROW ? 128 : TILE_CAP;            // staged candidates per pass
+    constexpr int NREC = ROW ? CAP + 4 : CAP;
     __shared__ float4 crec[TILE_MAXC];         // kept cells: origin in the tile frame, slot - index
     __shared__ int cupk[TILE_MAXC];            // kept cells: dx | dy << 8 | dz << 16
     __shared__ int sflag[64];                  // cells starting at a lane's candidate this round
-    __shared__ float4 sA[2 * (TILE_CAP + 1)];  // per candidate: (-2 k2 p, k2 |p|^2) tile-centred,
-                                               // then (cos, sin) of g.r, slot, cell offsets; the
-                                               // last record is a sentinel that is never a member
-    __shared__ unsigned short sub[SUB ? 8 : 1][SUB ? TILE_CAP : 1];   // per-group candidate lists
+    __shared__ float4 sA[REC * NREC];
     int* const cstart = reinterpret_cast<int*>(sA);    // kept cells' first candidate (before staging)
-    static_assert(sizeof(int) * TILE_MAXC <= sizeof(float4) * 2 * TILE_CAP, "cstart overlay");
+    static_assert(sizeof(int) * TILE_MAXC <= sizeof(float4) * REC * NREC, "cstart overlay");
+    static_assert(!ROW || 3 * 256 * sizeof(float) <= sizeof(float4) * REC * NREC, "row sums overlay");
 
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const unsigned long long le = lt | (1ull << lane);
@@ -1213,10 +1248,6 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             T += __shfl(incl, 63, 64);
         }
     }
-    if (lane == 0) {
-        sA[2 * TILE_CAP] = make_float4(0.0f, 0.0f, 0.0f, -__builtin_inff());
-        sA[2 * TILE_CAP + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
     __syncthreads();
     T = __builtin_amdgcn_readfirstlane(T);
     K = __builtin_amdgcn_readfirstlane(K);
@@ -1226,12 +1257,13 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     __syncthreads();
 
     // exact FP64 test of neighbourList1b (Lib_LinkCell3D.f90:1565-1596) for
-    // a staged candidate: packed unwrapped cell offsets u, point slot
-    auto exact_member = [&](int u, int slot) -> bool {
-        const int kx = (u & 255) - 1 - lcx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
+    // a staged candidate (packed unwrapped cell offsets u, point slot) and
+    // the voxel of x index vx (cell offset lx) in the lane's row
+    auto exact_at = [&](int lx, int vx, int u, int slot) -> bool {
+        const int kx = (u & 255) - 1 - lx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
         if (kx < -1 || kx > 1 || ky < -1 || ky > 1 || kz < -1 || kz > 1) return false;
         const double4 f = frac[slot];
-        const double zix = P.ax.zi[0][jx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
+        const double zix = P.ax.zi[0][vx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
         const double d1 = P.a[0] * (f.x + (kx - zix));
         const double d2 = P.a[1] * (f.y + (ky - ziy));
         const double d3 = P.a[2] * (f.z + (kz - ziz));
@@ -1247,12 +1279,15 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     float emax = -3.4028234663852886e38f;
     int cnt = 0;
     unsigned nguard = 0;
+    // ROW: sums and largest u of the row's four voxels over this quarter
+    float rre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float umax[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
 
     int t_next = 0, kbase = 0;
     while (t_next < T) {
         // stage: compact candidates within 3 sigma of the tile box
         int n = 0;
-        while (t_next < T && n <= TILE_CAP - 64) {
+        while (t_next < T && n <= CAP - 64) {
             // cell of candidate t_next + lane
             sflag[lane] = 0;
             __syncthreads();
@@ -1264,7 +1299,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             kbase += (int)__popcll(sm);
             const int t = t_next + lane;
             bool keep = false;
-            float4 rec, cs;
+            float4 r0, r1, r2;
             if (t < T) {
                 const float4 cr = crec[k];
                 const int slot = t + __float_as_int(cr.w);
@@ -1276,81 +1311,58 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= F.rc_cull2;
                 if (keep) {
                     const float m2k = -2.0f * k2;
-                    rec = make_float4(m2k * px, m2k * py, m2k * pz, k2 * fmaf(px, px, fmaf(py, py, pz * pz)));
-                    cs = make_float4(pc.w, sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]));
+                    if (ROW) {
+                        float xv[4];
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const float d = px - qxt[i];
+                            xv[i] = __builtin_amdgcn_exp2f(k2 * (d * d));
+                        }
+                        r0 = make_float4(xv[0], xv[1], xv[2], xv[3]);
+                        r1 = make_float4(m2k * py, m2k * pz, k2 * fmaf(py, py, pz * pz), pc.w);
+                        r2 = make_float4(sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]), 0.0f);
+                    } else {
+                        r0 = make_float4(m2k * px, m2k * py, m2k * pz, k2 * fmaf(px, px, fmaf(py, py, pz * pz)));
+                        r1 = make_float4(pc.w, sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]));
+                    }
                 }
             }
             const unsigned long long m = __ballot(keep);
             if (keep) {
                 const int p = n + (int)__popcll(m & lt);
-                sA[2 * p] = rec;
-                sA[2 * p + 1] = cs;
+                sA[REC * p] = r0;
+                sA[REC * p + 1] = r1;
+                if (ROW) sA[REC * p + 2] = r2;
             }
             n += (int)__popcll(m);
             t_next += 64;
         }
         n = __builtin_amdgcn_readfirstlane(n);
+        if (ROW && lane < 4) {
+            // zero records after the last candidate: X = 0, never a member
+            // and never in the guard band (u = -1)
+            const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sA[REC * (n + lane)] = z4;
+            sA[REC * (n + lane) + 1] = z4;
+            sA[REC * (n + lane) + 2] = z4;
+        }
         __syncthreads();
-        int len = n;                            // candidates this lane walks
-        if (SUB) {
-            // group voxel box (xor-shuffles inside the 8-lane group)
-            float gx0 = qcx, gx1 = qcx, gy0 = qcy, gy1 = qcy, gz0 = qcz, gz1 = qcz;
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1) {
-                gx0 = fminf(gx0, __shfl_xor(gx0, o, 64)); gx1 = fmaxf(gx1, __shfl_xor(gx1, o, 64));
-                gy0 = fminf(gy0, __shfl_xor(gy0, o, 64)); gy1 = fmaxf(gy1, __shfl_xor(gy1, o, 64));
-                gz0 = fminf(gz0, __shfl_xor(gz0, o, 64)); gz1 = fmaxf(gz1, __shfl_xor(gz1, o, 64));
-            }
-            // (-2 k2 p) / (-2 k2) = p
-            const float inv = 1.0f / (-2.0f * k2);
-            int gcnt = 0;
-            for (int j0 = 0; j0 < n; j0 += 8) {
-                const int j = j0 + pos;
-                bool keep = false;
-                if (j < n) {
-                    const float4 A = sA[2 * j];
-                    const float px = A.x * inv, py = A.y * inv, pz = A.z * inv;
-                    const float ex = fmaxf(0.0f, fmaxf(gx0 - px, px - gx1));
-                    const float ey = fmaxf(0.0f, fmaxf(gy0 - py, py - gy1));
-                    const float ez = fmaxf(0.0f, fmaxf(gz0 - pz, pz - gz1));
-                    keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= F.rc_cull2;
-                }
-                const unsigned gb = (unsigned)(__ballot(keep) >> (8 * grp)) & 0xFFu;
-                if (keep) sub[SUB ? grp : 0][gcnt + __popc(gb & ((1u << pos) - 1u))] = (unsigned short)j;
-                gcnt += __popc(gb);
-            }
-            len = gcnt;
-            __syncthreads();
-        }
-        int lmax = len;
-        if (SUB) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o, 64));
-            lmax = __builtin_amdgcn_readfirstlane(lmax);
-            if (!DIAG) {
-                // pad every group list to the wave's longest with the sentinel
-                for (int k = len + pos; k < lmax; k += 8) sub[SUB ? grp : 0][k] = (unsigned short)TILE_CAP;
-                __syncthreads();
-            }
-        }
-        lmax = __builtin_amdgcn_readfirstlane(lmax);
         if (DIAG) {
             // every voxel against every staged candidate, per-pair exact
             // membership with the counts and nearest ranges of the diagnostics
-            for (int jb = 0; jb < lmax; jb += 16) {
-                const int je = min(jb + 16, lmax);
+            for (int jb = 0; jb < n; jb += 16) {
+                const int je = min(jb + 16, n);
                 float pr = 0.0f, pi = 0.0f;
-                for (int jj = jb; jj < je; jj++) {
-                    const bool valid = !SUB || jj < len;
-                    const int j = SUB ? (valid ? (int)sub[SUB ? grp : 0][jj] : 0) : jj;
-                    const float4 A = sA[2 * j];
-                    const float4 C = sA[2 * j + 1];
+                for (int j = jb; j < je; j++) {
+                    const float4 A = sA[REC * j];
+                    const float4 C = sA[REC * j + 1];
                     const float e = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2)));
-                    bool in = valid && act && e >= F.e_rc;
-                    if (valid && act && fabsf(e - F.e_rc) <= F.e_band) {
+                    bool in = act && e >= F.e_rc;
+                    if (act && fabsf(e - F.e_rc) <= F.e_band) {
                         nguard++;
-                        in = exact_member(__float_as_int(C.w), __float_as_int(C.z));
+                        in = exact_at(lcx, jx, __float_as_int(C.w), __float_as_int(C.z));
                     }
+                    if (F.thin_z) in = in && (unsigned)((__float_as_int(C.w) >> 16) - lcz) <= 2u;
                     const float w = in ? __builtin_amdgcn_exp2f(e) : 0.0f;
                     pr = fmaf(w, C.x, pr);
                     pi = fmaf(w, C.y, pi);
@@ -1360,21 +1372,20 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 kahan_add(sre, cre, pr);
                 kahan_add(sim, cim, pi);
             }
-        } else {
+        } else if (!ROW) {
             // Branch-free pass over blocks of 32 candidates in the frame
             // t = e - k2 (3 sigma)^2 (member iff t >= 0, weight 2^t scaled by
             // 2^-e_rc), tracking the smallest |t|.  A block in which some
             // lane came within the guard band is summed again with the exact
             // FP64 test for those pairs (uniform branch, rare).
             const float kq2r = kq2 - F.e_rc;
-            for (int jb = 0; jb < lmax; jb += 32) {
-                const int je = min(jb + 32, lmax);
+            for (int jb = 0; jb < n; jb += 32) {
+                const int je = min(jb + 32, n);
                 float pr = 0.0f, pi = 0.0f, dmin = 3.4028234663852886e38f;
 #pragma unroll 4
-                for (int jj = jb; jj < je; jj++) {
-                    const int j = SUB ? (int)sub[SUB ? grp : 0][jj] : jj;
-                    const float4 A = sA[2 * j];
-                    const float4 C = sA[2 * j + 1];
+                for (int j = jb; j < je; j++) {
+                    const float4 A = sA[REC * j];
+                    const float4 C = sA[REC * j + 1];
                     const float t = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2r)));
                     const float w = t >= 0.0f ? __builtin_amdgcn_exp2f(t) : 0.0f;
                     pr = fmaf(w, C.x, pr);
@@ -1384,15 +1395,14 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 }
                 if (__ballot(dmin <= F.e_band)) {
                     pr = 0.0f; pi = 0.0f;
-                    for (int jj = jb; jj < je; jj++) {
-                        const int j = SUB ? (int)sub[SUB ? grp : 0][jj] : jj;
-                        const float4 A = sA[2 * j];
-                        const float4 C = sA[2 * j + 1];
+                    for (int j = jb; j < je; j++) {
+                        const float4 A = sA[REC * j];
+                        const float4 C = sA[REC * j + 1];
                         const float t = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2r)));
                         bool in = t >= 0.0f;
                         if (fabsf(t) <= F.e_band) {
                             nguard++;
-                            in = exact_member(__float_as_int(C.w), __float_as_int(C.z));
+                            in = exact_at(lcx, jx, __float_as_int(C.w), __float_as_int(C.z));
                         }
                         const float w = in ? __builtin_amdgcn_exp2f(t) : 0.0f;
                         pr = fmaf(w, C.x, pr);
@@ -1402,8 +1412,96 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 kahan_add(sre, cre, pr);
                 kahan_add(sim, cim, pi);
             }
+        } else {
+            // ROW: steps of four candidates (one per quarter), blocks of 8
+            // steps; the guard band in u is the exponent's band (|u| ~ ln2 |t|
+            // near the threshold, so it is 1.44 times as wide)
+            const int nst = (n + 3) >> 2;
+            for (int kb = 0; kb < nst; kb += 8) {
+                const int ke = min(kb + 8, nst);
+                float dmin = 3.4028234663852886e38f;
+#pragma unroll 2
+                for (int kk = kb; kk < ke; kk++) {
+                    const int j = 4 * kk + squart;
+                    const float4 X = sA[REC * j];
+                    const float4 B = sA[REC * j + 1];
+                    float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
+                    float sn;
+                    if (ROW == 2) {
+                        const float4 C = sA[REC * j + 2];
+                        sn = C.x;
+                        E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
+                    } else {
+                        sn = sA[REC * j + 2].x;
+                    }
+                    const float ec = E * B.w, es = E * sn;
+                    const float xv[4] = {X.x, X.y, X.z, X.w};
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float u = fmaf(xv[i], E, -1.0f);
+                        const float xs = u >= 0.0f ? xv[i] : 0.0f;
+                        rre[i] = fmaf(xs, ec, rre[i]);
+                        rim[i] = fmaf(xs, es, rim[i]);
+                        dmin = fminf(dmin, fabsf(u));
+                        if (DENSITY) umax[i] = fmaxf(umax[i], u);
+                    }
+                }
+                if (__ballot(dmin <= F.e_band)) {
+                    for (int kk = kb; kk < ke; kk++) {
+                        const int j = 4 * kk + squart;
+                        const float4 X = sA[REC * j];
+                        const float4 B = sA[REC * j + 1];
+                        const float4 C = sA[REC * j + 2];
+                        float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
+                        if (ROW == 2) E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
+                        const float ec = E * B.w, es = E * C.x;
+                        const float xv[4] = {X.x, X.y, X.z, X.w};
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const float u = fmaf(xv[i], E, -1.0f);
+                            if (fabsf(u) <= F.e_band) {
+                                const int vx = min(x0 + i, x1);
+                                if (row_act && x0 + i <= x1) nguard++;
+                                const bool in = exact_at(P.ax.ql[0][vx].lc, vx, __float_as_int(C.z),
+                                                         __float_as_int(C.y));
+                                if (in != (u >= 0.0f)) {
+                                    const float d = in ? xv[i] : -xv[i];
+                                    rre[i] = fmaf(d, ec, rre[i]);
+                                    rim[i] = fmaf(d, es, rim[i]);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
         }
         __syncthreads();
+    }
+
+    if (ROW) {
+        // quarters' sums of each voxel, added in quarter order; lane (row, s)
+        // takes voxel s of its row
+        float2* red = reinterpret_cast<float2*>(sA);                  // [64 lanes][4 voxels]
+        float* redm = reinterpret_cast<float*>(sA) + 2 * 256;         // [64 lanes][4 voxels]
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            red[4 * lane + i] = make_float2(rre[i], rim[i]);
+            if (DENSITY) redm[4 * lane + i] = umax[i];
+        }
+        __syncthreads();
+        const int rb = lane & ~3;
+        float a = 0.0f, b = 0.0f, um = -1.0f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float2 v = red[4 * (rb + q) + squart];
+            a += v.x;
+            b += v.y;
+            if (DENSITY) um = fmaxf(um, redm[4 * (rb + q) + squart]);
+        }
+        sre = a;
+        sim = b;
+        // largest exponent in the t frame: 2^t = u + 1 (u = -1: no candidate)
+        if (DENSITY && um > -1.0f) emax = __builtin_amdgcn_logf(um + 1.0f);
     }
 
     if (act) {
@@ -1439,8 +1537,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             out_r2m[o] = r2m;
         }
     }
-    // spread over GUARD_SLOTS lines: many waves adding into one address serialise
-    if (act && nguard) atomicAdd(guard_count + GUARD_STRIDE * (blockIdx.x & (GUARD_SLOTS - 1)), (unsigned long long)nguard);
+    // spread over GUARD_SLOTS lines: many waves adding into one address
+    // serialise.  ROW lanes counted only their active voxels.
+    if ((ROW || act) && nguard)
+        atomicAdd(guard_count + GUARD_STRIDE * (blockIdx.x & (GUARD_SLOTS - 1)), (unsigned long long)nguard);
 }
 
 // ------------------------------------------------------------------ (c) integrate, FAST
@@ -1714,6 +1814,13 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     F.inv_k2 = (float)(1.0 / k2);
     F.rc_cull2 = (float)(p->rc2 * (1.0 + 1.0e-4));
     F.cellf[0] = (float)p->a_col[0]; F.cellf[1] = (float)p->a_col[4]; F.cellf[2] = (float)p->a_col[8];
+    // cells thinner than 3 sigma by more than rounding (the guard band, ~2e-5
+    // A, covers less): only z can be, x and y are wide by construction
+    const double rc = std::sqrt(p->rc2), thin = rc * (1.0 - 1.0e-7);
+    if (p->a_col[0] < thin || p->a_col[4] < thin)
+        return seterr(D2BI_ERR_ARG, "link cells %.6f x %.6f narrower than 3 sigma = %.6f", p->a_col[0],
+                      p->a_col[4], rc);
+    F.thin_z = p->a_col[8] < thin ? 1 : 0;
     // the unwrapped cell range of a tile must fit the per-tile cell table
     int worst = 1;
     for (int a = 0; a < 3; a++) {
@@ -1727,22 +1834,37 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     return D2BI_OK;
 }
 
+// Kernel instance for (diagnostics, density, variant); link cells thinner
+// than 3 sigma along z take the row kernel with the z-offset test.
+template <class Go>
+static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
+{
+    if (diag) { go(k_voxel_tile<true, true, 0>); return; }
+    const int row = thin_z ? 2 : (variant ? 1 : 0);
+    if (dens) {
+        if (row == 2) go(k_voxel_tile<true, false, 2>);
+        else if (row == 1) go(k_voxel_tile<true, false, 1>);
+        else go(k_voxel_tile<true, false, 0>);
+    } else {
+        if (row == 2) go(k_voxel_tile<false, false, 2>);
+        else if (row == 1) go(k_voxel_tile<false, false, 1>);
+        else go(k_voxel_tile<false, false, 0>);
+    }
+}
+
 template <bool DIAG>
 static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start, const double4* frac,
                         const float4* p32, const float* sn32, float4* field, int* cnt, float* r2m,
                         unsigned long long* gc, hipStream_t s)
 {
-    auto go = [&](auto kern) {
+    pick_tile(DIAG, dens, variant, F.thin_z != 0, [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, frac, p32, sn32, field, cnt,
                            r2m, gc);
-    };
-    if (DIAG) { if (variant) go(k_voxel_tile<true, true, true>); else go(k_voxel_tile<true, true, false>); }
-    else if (dens) { if (variant) go(k_voxel_tile<true, false, true>); else go(k_voxel_tile<true, false, false>); }
-    else { if (variant) go(k_voxel_tile<false, false, true>); else go(k_voxel_tile<false, false, false>); }
+    });
 }
 
-// Choice of the FAST voxel kernel variant (0: one candidate list per 4x4x4
-// tile, 1: per 2x2x2 group) per voxel-grid shape: timed once on the first
+// Choice of the FAST voxel kernel variant (0: a lane per voxel, 1: a lane
+// per row of four voxels and candidate quarter) per voxel-grid shape: timed once on the first
 // condition of that shape, then cached.  D2BI_FAST_VARIANT=0|1 forces one.
 static std::map<std::array<int, 7>, int> g_variant_cache;
 
@@ -1769,7 +1891,7 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, TimeFn time_va
     auto& cache = g_variant_cache;
     const int f = forced_variant();
     if (f >= 0) return f;
-    if (diag) return 1;
+    if (diag) return 0;
     const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
@@ -1827,12 +1949,9 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
                            (float*)dr2, gp);
     };
     const bool dens = r->use_density != 0;
-    auto run_variant = [&](int v) {
-        if (want_diag) { if (v) launch(k_voxel_tile<true, true, true>); else launch(k_voxel_tile<true, true, false>); }
-        else if (dens) { if (v) launch(k_voxel_tile<true, false, true>); else launch(k_voxel_tile<true, false, false>); }
-        else { if (v) launch(k_voxel_tile<false, false, true>); else launch(k_voxel_tile<false, false, false>); }
-    };
-    const int variant = fast_variant(p, dens, want_diag, [&](int v) -> float {
+    auto run_variant = [&](int v) { pick_tile(want_diag, dens, v, F.thin_z != 0, launch); };
+    // thin z cells have one kernel (no tuning)
+    const int variant = F.thin_z ? 1 : fast_variant(p, dens, want_diag, [&](int v) -> float {
         HIPCHK(hipEventRecord(g.ev[7], s));
         run_variant(v);
         HIPCHK(hipEventRecord(g.ev[8], s));
@@ -2450,7 +2569,10 @@ int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t 
     CHK(upload_axis(tab, &ax, s));
     Binned b;
     CHK(run_binning(d_x, nAtoms, BP, &b, s));
-    CHK(run_voxel(rec, &plan, tab, ax, mode, true, nullptr, s));
+    // FAST without count and r2min runs the imaging kernel itself (the
+    // variant D2BI_FAST_VARIANT names, or the tuned one)
+    const bool diag = mode == D2BI_MODE_FP64 || count || r2min;
+    CHK(run_voxel(rec, &plan, tab, ax, mode, diag, nullptr, s));
     HIPCHK(hipStreamSynchronize(s));
     const int64_t nvox = (int64_t)plan.Mx * plan.My * plan.Mz;
     Workspace& w = g.ws;
@@ -2464,11 +2586,14 @@ int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t 
         if (rho) HIPCHK(hipMemcpy(rho, w.f64_rho.p, sizeof(double) * nvox, hipMemcpyDeviceToHost));
         if (r2min) HIPCHK(hipMemcpy(r2min, w.r2m.p, sizeof(double) * nvox, hipMemcpyDeviceToHost));
     } else {
-        std::vector<float> f(4 * nvox);
-        HIPCHK(hipMemcpy(f.data(), w.f32_field.p, sizeof(float) * 4 * nvox, hipMemcpyDeviceToHost));
+        // (re, im, rho, 0) per voxel, or (re, im) from the imaging kernel
+        // without the density field (rho = 1)
+        const int nc = (!diag && !rec->use_density) ? 2 : 4;
+        std::vector<float> f(nc * nvox);
+        HIPCHK(hipMemcpy(f.data(), w.f32_field.p, sizeof(float) * nc * nvox, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < nvox; i++) {
-            if (phase_re_im) { phase_re_im[2 * i] = f[4 * i]; phase_re_im[2 * i + 1] = f[4 * i + 1]; }
-            if (rho) rho[i] = f[4 * i + 2];
+            if (phase_re_im) { phase_re_im[2 * i] = f[nc * i]; phase_re_im[2 * i + 1] = f[nc * i + 1]; }
+            if (rho) rho[i] = nc == 4 ? f[4 * i + 2] : 1.0f;
         }
         if (r2min) {
             std::vector<float> r(nvox);

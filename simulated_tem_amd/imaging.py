@@ -250,21 +250,24 @@ class DynamicalTwoBeamImaging:
                                    int(nPrecessionAngle), m, ptr(img), C.byref(st)))
         return img
 
-    def field(self, x, mode="fp64"):
-        """Voxel field for tests: (phase[Mz,My,Mx] complex, rho, count, r2min)."""
+    def field(self, x, mode="fp64", diag=True):
+        """Voxel field for tests: (phase[Mz,My,Mx] complex, rho, count, r2min).
+        diag=False (FAST) takes the field from the imaging kernel itself and
+        returns None for count and r2min."""
         _ensure()
         p = self.plan(len(x))
         n = p.Mx * p.My * p.Mz
         ph = np.zeros(2 * n)
         rho = np.zeros(n)
-        cnt = np.zeros(n, np.int32)
-        r2m = np.zeros(n)
+        cnt = np.zeros(n, np.int32) if diag else None
+        r2m = np.zeros(n) if diag else None
         x = f64(x)
         check(abi.load().d2bi_gpu_field(C.byref(self.rec), ptr(x), 3, len(x), MODES[mode], ptr(ph),
-                                        ptr(rho), iptr(cnt), ptr(r2m)))
+                                        ptr(rho), iptr(cnt) if diag else None,
+                                        ptr(r2m) if diag else None))
         shp = (p.Mz, p.My, p.Mx)
-        return ((ph[0::2] + 1j * ph[1::2]).reshape(shp), rho.reshape(shp), cnt.reshape(shp),
-                r2m.reshape(shp))
+        return ((ph[0::2] + 1j * ph[1::2]).reshape(shp), rho.reshape(shp),
+                cnt.reshape(shp) if diag else None, r2m.reshape(shp) if diag else None)
 
     def linkcells(self, x):
         """findGdotR's link cells: (cell_start, ids, frac, gdotr)."""

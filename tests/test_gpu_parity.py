@@ -49,6 +49,11 @@ CASES = {
     "c2_small_g200_density": lambda: _cond("c2", cells=(14, 12, 30), sg=-0.02),
     "c0_k111_g1m10": lambda: _cond("c0", cells=(9, 11, 7), g=(1, -1, 0), k=(1, 1, 1),
                                    d=(25.0, 27.0, 19.0)),
+    # link cells thinner than 3 sigma along z (4.725 A, Ez = 8), and a foil
+    # of three cells (Ez = 3, 2.63 A): points two cells away in z can lie
+    # within 3 sigma and neighbourList1b leaves them out
+    "c0_thin_z": lambda: _cond("c0", cells=(12, 12, 12), d=(37.9824, 37.9824, 37.8)),
+    "c0_foil_ez3": lambda: _cond("c0", cells=(12, 12, 3), d=(37.9824, 37.9824, 7.9)),
 }
 
 
@@ -86,13 +91,30 @@ def test_field_fp64_membership_bitexact(case):
 
 @pytest.fixture(params=["0", "1"])
 def fast_variant(request, monkeypatch):
-    """Both FAST voxel kernels: one candidate list per 4x4x4 tile (0) and
-    per 2x2x2 group (1)."""
+    """Both FAST voxel kernels: a lane per voxel (0) and a lane per row of
+    four voxels and quarter of the candidates (1)."""
     monkeypatch.setenv("D2BI_FAST_VARIANT", request.param)
     return request.param
 
 
-def test_field_fast_membership_exact(case, fast_variant):
+def test_field_fast_imaging_kernel(case, fast_variant):
+    """The imaging kernel's own field (no diagnostics) against the oracle.
+    Leaving out or adding one atom at the 3 sigma cut moves x by its weight
+    exp(-4.5) over |S| (~2e-3 for bcc), far above the FP32 error (~3e-7), so
+    the bound below also checks that membership is exact."""
+    name, x, im, oc = case
+    ph, rho, _, _ = im.field(x, "fast", diag=False)
+    oph, orho, ocnt, or2m = oc.field(nthreads=NTH)
+    err = np.abs(ph - oph)
+    print("[fast field, imaging kernel] %s v%s max|dx| %.3e mean %.3e max|drho| %.3e"
+          % (name, fast_variant, err.max(), err.mean(), np.max(np.abs(rho - orho))), flush=True)
+    assert err.max() < 1e-5
+    assert np.max(np.abs(rho - orho)) < 1e-5
+
+
+def test_field_fast_membership_exact(case):
+    """The FAST kernel's diagnostic form (a lane per voxel, neighbour counts
+    and nearest ranges): counts identical to the oracle's."""
     name, x, im, oc = case
     ph, rho, cnt, r2m = im.field(x, "fast")
     oph, orho, ocnt, or2m = oc.field(nthreads=NTH)
@@ -100,8 +122,7 @@ def test_field_fast_membership_exact(case, fast_variant):
     occ = ocnt > 0
     assert np.max(np.abs(r2m[occ] - or2m[occ]) / np.maximum(or2m[occ], 1.0)) < 1e-5
     err = np.abs(ph - oph)
-    print("[fast field] %s v%s max|dx| %.3e mean %.3e" % (name, fast_variant, err.max(), err.mean()),
-          flush=True)
+    print("[fast field] %s max|dx| %.3e mean %.3e" % (name, err.max(), err.mean()), flush=True)
     assert np.max(np.abs(rho - orho)) < 1e-5
 
 
