@@ -30,6 +30,7 @@ extern "C" {
 
 #define D2BI_MAX_UVW   125      /* Lib_DynamicalTwoBeamImaging.f90:86 uvw(3,125)          */
 #define D2BI_MAX_PREC  256      /* precession points per condition                        */
+#define D2BI_MAX_APERTURE 1024 /* aperture points per condition                         */
 
 enum {
     D2BI_OK            =  0,
@@ -156,6 +157,20 @@ int d2bi_setup_tilt_ng(d2bi_record* rec, const double k[3], double ng, int32_t l
 int d2bi_rotate_to_standard_frame(const double k[3], const double a_cell[9],
                                   const double hkl[3], double U[9]);
 
+/* evenlySpacedPointsInCircle(x) (Lib_EvenlySpacedPointsInCircle.f90:29-100,
+ * no file): n points in the unit disk, the reference's table of known
+ * arrangements for n <= 25 (src/evenlySpaced.h, FP32 literals) and the
+ * sunflower placement beyond.  xy is (2, n), x first. */
+int d2bi_aperture_points(int32_t n, double* xy);
+
+/* The aperture sampling of computeD2BI_img's optional apertureAngle and
+ * nAperturePoints (Lib_DynamicalTwoBeamImaging.f90:1655-1667, 1805-1836,
+ * findGdotR_aperture :1336-1344): gshift (3, nAperture) the g-vector used for
+ * g.r at each aperture point, aa (nAperture, nPrec) 2 pi s_g of each
+ * (aperture point, precession point), column-major. */
+int d2bi_aperture_setup(const d2bi_record* rec, double precAngle, int32_t nPrec, double apertureAngle,
+                        int32_t nAperture, double* gshift, double* aa);
+
 /* wavelength / velocity (Lib_RelativisticElectrons.f90:24-50), V in volts. */
 double d2bi_wavelength(double V);
 double d2bi_velocity(double V);
@@ -186,6 +201,17 @@ int64_t d2bi_gpu_atoms_count(d2bi_atoms atoms);
 int d2bi_gpu_image(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms,
                    double precAngle, int32_t nPrec, int32_t mode,
                    double* img, d2bi_stats* stats);
+
+/* computeD2BI_img(this,x,d2big2,precessionAngle,nPrecessionAngle,
+ *                 apertureAngle,nAperturePoints) (Lib_DynamicalTwoBeamImaging.f90:1563)
+ * with the aperture sampled at nAperture points (radians; nAperture <=
+ * D2BI_MAX_APERTURE): the mean over nAperture x nPrec integrations, each
+ * aperture point with its own g.r (findGdotR_aperture :1269-1396).  The d2bi
+ * command line keeps nAperture = 1 (d2bi.f90:92, 378-379); nAperture = 1
+ * gives the image of d2bi_gpu_image. */
+int d2bi_gpu_image_aperture(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms,
+                            double precAngle, int32_t nPrec, double apertureAngle, int32_t nAperture,
+                            int32_t mode, double* img, d2bi_stats* stats);
 
 /* Same, with device-resident atoms: the timed region of a benchmark. */
 int d2bi_gpu_image_dev(const d2bi_record* rec, d2bi_atoms atoms,

@@ -30,6 +30,15 @@
  *   orc_blur                  constant image invariance, delta response
  *                              = normalised separable Gaussian, symmetry
  *   orc_plan                  golden .dat header (|g|, scale, image space)
+ *   orc_sunflower             radii sqrt((2k-1)/(2n-1)) and golden-angle steps
+ *                              (closed form), all points inside the unit disk
+ *   orc_aperture_setup        nApp = 1 and apertureAngle = 0 reduce to orc_plan's
+ *                              aa; |gshift - g| = modg * |point|, shift in x, y only
+ *   orc_compute_img_aperture  nApp = 1, and apertureAngle = 0 for any nApp,
+ *                              reproduce orc_compute_img; g.r per point differs
+ *                              from the unshifted one by dg.y exactly
+ *                              (the 25 tabulated point sets are the fixture
+ *                              tests/golden/aperture_points.json)
  */
 #include <math.h>
 #include <complex.h>
@@ -706,6 +715,200 @@ int orc_compute_img(const d2bi_record* r, const double* x, int64_t ld, int64_t n
     orc_image_window(s, r, 0, s->plan.Mx, 0, s->plan.My, img, nthreads);
     orc_state_free(s);
     return 0;
+}
+
+/* ------------------------------------------------------------------ aperture */
+
+/* sunflower(n, alpha=0) with radius(k, n, b)
+ * (Lib_EvenlySpacedPointsInCircle.f90:438-478), that module's
+ * PI = 3.141592654; the point set evenlySpacedPointsInCircle returns for
+ * n > 25 (:93-97).  xy is (2, n). */
+void orc_sunflower(int n, double* xy)
+{
+    const double pi = 3.141592654;
+    const double phi = (sqrt(5.0) + 1.0) / 2;
+    const double alpha = 0.0;
+    const int bb = (int)lround(sqrt(alpha * alpha * n));
+    for (int kk = 1; kk <= n; kk++) {
+        double rr;
+        if (kk > n - bb) rr = 1;
+        else rr = sqrt((double)(2 * kk - 1) / (2 * n - bb - 1));
+        double theta = 2 * pi * kk / (phi * phi);
+        xy[2 * (kk - 1)] = rr * cos(theta);
+        xy[2 * (kk - 1) + 1] = rr * sin(theta);
+    }
+}
+
+/* computeD2BI_img aperture sampling (Lib_DynamicalTwoBeamImaging.f90:1655-1667,
+ * 1805-1836) and findGdotR_aperture's shifted g-vectors (:1336-1344).
+ * xy: the nApp unit-disk points (2, nApp).  gshift (3, nApp); aa (nApp, nPrec),
+ * app fastest. */
+int orc_aperture_setup(const d2bi_record* r, double precAngle, int nPrec, double apertureAngle, int nApp,
+                       const double* xy, double* gshift, double* aa)
+{
+    if (nPrec < 1 || nPrec > D2BI_MAX_PREC || nApp < 1) return -1;
+    double modk = 2 * PI_IMG / orc_wavelength(r->E * 1000);
+    double modg = apertureAngle * modk;
+    double gg0[3];
+    get_aperture(r, NULL, gg0);
+    for (int app = 0; app < nApp; app++) {
+        double dgx = modg * xy[2 * app], dgy = modg * xy[2 * app + 1];
+        gshift[3 * app] = gg0[0] + dgx;
+        gshift[3 * app + 1] = gg0[1] + dgy;
+        gshift[3 * app + 2] = gg0[2];
+    }
+    double costheta = 1.0, sintheta = 0.0;
+    if (nPrec > 1) { costheta = cos(precAngle); sintheta = sin(precAngle); }
+    for (int prec = 1; prec <= nPrec; prec++) {
+        double gg[3];
+        if (prec == 1) {
+            get_aperture(r, NULL, gg);
+        } else {
+            double phi = (prec - 1) * 2 * PI_IMG / (nPrec - 1);
+            double ux = sin(phi), uy = cos(phi), T[9];
+            M3(T,1,1) = costheta + ux * ux * (1 - costheta); M3(T,1,2) = ux * uy * (1 - costheta);             M3(T,1,3) = uy * sintheta;
+            M3(T,2,1) = ux * uy * (1 - costheta);             M3(T,2,2) = costheta + uy * uy * (1 - costheta); M3(T,2,3) = -ux * sintheta;
+            M3(T,3,1) = -uy * sintheta;                       M3(T,3,2) = ux * sintheta;                       M3(T,3,3) = costheta;
+            get_aperture(r, T, gg);
+        }
+        for (int app = 0; app < nApp; app++) {
+            double dg[3] = {modg * xy[2 * app], modg * xy[2 * app + 1], modg * 0.0};
+            double g2[3] = {gg[0] + dg[0], gg[1] + dg[1], gg[2] + dg[2]};
+            aa[app + nApp * (prec - 1)] = 2 * PI_IMG * orc_deviation_parameter(r->E, g2);
+        }
+    }
+    return 0;
+}
+
+/* findGdotR_aperture (Lib_DynamicalTwoBeamImaging.f90:1269-1396): as
+ * orc_find_gdotr, with gdotr(ii, kk) = gshift(:,kk).yy for every aperture
+ * point; gdotr is (cap, nApp) column-major. */
+int64_t orc_find_gdotr_aperture(const d2bi_record* r, const d2bi_plan* p, const double* x, int64_t ld,
+                                int64_t nAtoms, orc_lc* lc, const double* gshift, int nApp, double* gdotr,
+                                int64_t cap)
+{
+    orc_lc_clear(lc);
+    const double* RU = p->RU;
+    int64_t ii = 0;
+    for (int nn = 0; nn < r->nuvw; nn++) {
+        double Ruvw[3];
+        const int* u = &r->uvw[3 * nn];
+        for (int i = 0; i < 3; i++) {
+            double s1 = r->super_a[i] * r->super_n[0];
+            double s2 = r->super_a[3 + i] * r->super_n[1];
+            double s3 = r->super_a[6 + i] * r->super_n[2];
+            Ruvw[i] = (u[0] * s1 + u[1] * s2) + u[2] * s3;
+        }
+        for (int64_t jj = 0; jj < nAtoms; jj++) {
+            const double* xa = &x[ld * jj];
+            double y0[3], yy[3], zz[3];
+            for (int i = 0; i < 3; i++) y0[i] = xa[i] + Ruvw[i];
+            for (int i = 1; i <= 3; i++)
+                yy[i - 1] = (M3(RU,i,1) * y0[0] + M3(RU,i,2) * y0[1]) + M3(RU,i,3) * y0[2];
+            for (int i = 0; i < 3; i++) { yy[i] = yy[i] + r->delta[i]; zz[i] = yy[i] - p->lc_delta[i]; }
+            if (point_in_box(lc->ia, lc->Nx, lc->Ny, lc->Nz, zz)) {
+                ii++;
+                if (ii > cap) return -2;
+                if (orc_lc_add(lc, (int)ii, zz)) return -1;
+                for (int kk = 0; kk < nApp; kk++) {
+                    const double* g = &gshift[3 * kk];
+                    gdotr[(ii - 1) + cap * kk] = (g[0] * yy[0] + g[1] * yy[1]) + g[2] * yy[2];
+                }
+            }
+        }
+    }
+    return ii;
+}
+
+/* computeD2BI_img's nApp > 1 branch (:1905-1934) over a window of columns:
+ * per voxel one neighbour list, r2_min and rho, a phase sum per aperture
+ * point; per column the integrations in (prec outer, app inner) order,
+ * divided by nApp*nPrec after the sum (:1951).  gdotr as above. */
+void orc_image_aperture_window(const d2bi_record* r, const d2bi_plan* p, const orc_lc* lc, const double* gdotr,
+                               int64_t cap, int nApp, const double* aa, int ix0, int ix1, int iy0, int iy1,
+                               double* img, int nthreads)
+{
+    const int Mz = p->Mz, wx = ix1 - ix0;
+    int ncap = 27 * (p->nMax > 0 ? p->nMax : 1) * 16 + 64;
+    (void)nthreads;
+#ifdef _OPENMP
+    #pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    {
+        int* id = (int*)malloc(sizeof(int) * ncap);
+        double* r2 = (double*)malloc(sizeof(double) * ncap);
+        double* xr = (double*)malloc(sizeof(double) * (Mz + 1) * nApp);
+        double* xi = (double*)malloc(sizeof(double) * (Mz + 1) * nApp);
+        double* rh = (double*)malloc(sizeof(double) * (Mz + 1));
+        double complex* s = (double complex*)malloc(sizeof(double complex) * nApp);
+#ifdef _OPENMP
+        #pragma omp for schedule(dynamic, 1)
+#endif
+        for (int iy = iy0; iy < iy1; iy++)
+            for (int ix = ix0; ix < ix1; ix++) {
+                for (int iz = 0; iz <= Mz; iz++) rh[iz] = 1.0;
+                for (int iz = 0; iz < Mz; iz++) {
+                    double xx[3];
+                    for (int i = 0; i < 3; i++) {
+                        double c = (r->gk_a[i] * ix + r->gk_a[3 + i] * iy) + r->gk_a[6 + i] * iz;
+                        xx[i] = c - p->lc_delta[i];
+                    }
+                    int nn = orc_lc_neighbours(lc, xx, 3 * p->sigma, id, r2, NULL);
+                    double r2_min = 3.4028234663852886e38;
+                    for (int app = 0; app < nApp; app++) s[app] = 0.0;
+                    for (int kk = 0; kk < nn; kk++) {
+                        r2_min = r2_min < r2[kk] ? r2_min : r2[kk];
+                        for (int app = 0; app < nApp; app++)
+                            s[app] = s[app] + cexp(CMPLX(-r2[kk] * p->i2s2, gdotr[(id[kk] - 1) + cap * app]));
+                    }
+                    if (r->use_density) rh[iz] = orc_rho(r2_min, p->sigma);
+                    for (int app = 0; app < nApp; app++) {
+                        double ww = cabs(s[app]);
+                        ww = 1 / (1.0e-16 > ww ? 1.0e-16 : ww);
+                        xr[iz + (Mz + 1) * app] = creal(s[app]) * ww;
+                        xi[iz + (Mz + 1) * app] = cimag(s[app]) * ww;
+                    }
+                }
+                for (int app = 0; app < nApp; app++) {
+                    xr[Mz + (Mz + 1) * app] = xr[(Mz + 1) * app];
+                    xi[Mz + (Mz + 1) * app] = xi[(Mz + 1) * app];
+                }
+                rh[Mz] = rh[0];
+                double acc = 0.0;
+                for (int prec = 0; prec < p->nPrec; prec++)
+                    for (int app = 0; app < nApp; app++)
+                        acc = acc + orc_integrate_column(aa[app + nApp * prec], p->b0, p->bg, p->dz,
+                                                         xr + (Mz + 1) * app, xi + (Mz + 1) * app, rh, Mz);
+                img[(int64_t)(ix - ix0) + (int64_t)wx * (iy - iy0)] = acc / (nApp * p->nPrec);
+            }
+        free(id); free(r2); free(xr); free(xi); free(rh); free(s);
+    }
+}
+
+/* Whole aperture-sampled image (computeD2BI_img with apertureAngle,
+ * nAperturePoints = nApp); xy the unit-disk points.  Returns 0 or < 0. */
+int orc_compute_img_aperture(const d2bi_record* r, const double* x, int64_t ld, int64_t nAtoms,
+                             double precAngle, int nPrec, double apertureAngle, int nApp, const double* xy,
+                             double* img, int nthreads)
+{
+    d2bi_plan plan;
+    if (orc_plan(r, nAtoms, precAngle, nPrec, &plan)) return -1;
+    double* gshift = (double*)malloc(sizeof(double) * 3 * nApp);
+    double* aa = (double*)malloc(sizeof(double) * nApp * nPrec);
+    if (orc_aperture_setup(r, precAngle, nPrec, apertureAngle, nApp, xy, gshift, aa)) {
+        free(gshift); free(aa);
+        return -1;
+    }
+    orc_lc* lc = orc_lc_new(plan.a_col, plan.Ex, plan.Ey, plan.Ez, plan.nMax);
+    int64_t cap = (int64_t)r->nuvw * nAtoms + 1;
+    double* gdotr = (double*)malloc(sizeof(double) * cap * nApp);
+    int64_t np = orc_find_gdotr_aperture(r, &plan, x, ld, nAtoms, lc, gshift, nApp, gdotr, cap);
+    int rc = 0;
+    if (np < 0) rc = (int)np - 10;
+    else orc_image_aperture_window(r, &plan, lc, gdotr, cap, nApp, aa, 0, plan.Mx, 0, plan.My, img, nthreads);
+    orc_lc_free(lc);
+    free(gdotr); free(gshift); free(aa);
+    return rc;
 }
 
 /* gaussianBlurImage (d2bi.f90:1543-1624) */

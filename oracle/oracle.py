@@ -90,6 +90,16 @@ def lib():
         L.orc_compute_img.argtypes = [C.POINTER(D2biRecord), _dp, C.c_int64, C.c_int64,
                                       C.c_double, C.c_int, _dp, C.c_int]
         L.orc_blur.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _dp]
+        L.orc_sunflower.argtypes = [C.c_int, _dp]
+        L.orc_aperture_setup.restype = C.c_int
+        L.orc_aperture_setup.argtypes = [C.POINTER(D2biRecord), C.c_double, C.c_int, C.c_double, C.c_int,
+                                         _dp, _dp, _dp]
+        L.orc_find_gdotr_aperture.restype = C.c_int64
+        L.orc_find_gdotr_aperture.argtypes = [C.POINTER(D2biRecord), C.POINTER(D2biPlan), _dp, C.c_int64,
+                                              C.c_int64, C.c_void_p, _dp, C.c_int, _dp, C.c_int64]
+        L.orc_compute_img_aperture.restype = C.c_int
+        L.orc_compute_img_aperture.argtypes = [C.POINTER(D2biRecord), _dp, C.c_int64, C.c_int64, C.c_double,
+                                               C.c_int, C.c_double, C.c_int, _dp, _dp, C.c_int]
         _lib = L
     return _lib
 
@@ -277,6 +287,72 @@ class Condition:
 def compute_img(rec, x, prec_angle=0.0, n_prec=1, nthreads=1):
     """Whole image, shape (My, Mx) (row = iy, column = ix)."""
     return Condition(rec, x, prec_angle, n_prec).image(nthreads=nthreads)
+
+
+_APERTURE_FIXTURE = os.path.join(os.path.dirname(_HERE), "tests", "golden", "aperture_points.json")
+
+
+def aperture_points(n):
+    """evenlySpacedPointsInCircle (Lib_EvenlySpacedPointsInCircle.f90:29-100):
+    (n, 2).  n <= 25: the reference's table (the fixture, FP32 literals);
+    otherwise the sunflower restatement."""
+    n = int(n)
+    if n <= 25:
+        import json
+        pts = json.load(open(_APERTURE_FIXTURE))["points"][str(n)]
+        return np.array([[float(np.float32(a)), float(np.float32(b))] for a, b in pts])
+    xy = np.zeros(2 * n)
+    lib().orc_sunflower(n, _p(xy))
+    return xy.reshape(n, 2)
+
+
+def aperture_setup(rec, prec_angle, n_prec, ap_angle, n_app):
+    """(gshift (nApp,3), aa (nPrec, nApp)) of computeD2BI_img's aperture
+    sampling (Lib_DynamicalTwoBeamImaging.f90:1655-1667, 1805-1836)."""
+    xy = _f64(aperture_points(n_app).reshape(-1))
+    gs = np.zeros(3 * n_app)
+    aa = np.zeros(n_app * n_prec)
+    if lib().orc_aperture_setup(C.byref(rec), float(prec_angle), int(n_prec), float(ap_angle), int(n_app),
+                                _p(xy), _p(gs), _p(aa)):
+        raise ValueError("orc_aperture_setup failed")
+    return gs.reshape(n_app, 3), aa.reshape(n_prec, n_app)
+
+
+def gdotr_aperture(rec, x, gshift):
+    """findGdotR_aperture (:1269-1396) for arbitrary g-vectors gshift (k,3):
+    (n_points, k), rows in point order ii."""
+    x, ld = _atoms_colmajor(x)
+    plan = D2biPlan()
+    if lib().orc_plan(C.byref(rec), len(x), 0.0, 1, C.byref(plan)):
+        raise ValueError("orc_plan failed")
+    gs = _f64(np.asarray(gshift, float).reshape(-1))
+    k = len(gs) // 3
+    cap = rec.nuvw * len(x) + 1
+    out = np.zeros(cap * k)
+    h = lib().orc_lc_new(_p(_f64(np.array(plan.a_col))), plan.Ex, plan.Ey, plan.Ez, plan.nMax)
+    try:
+        n = lib().orc_find_gdotr_aperture(C.byref(rec), C.byref(plan), _p(x), ld, len(x), h, _p(gs), k,
+                                          _p(out), cap)
+    finally:
+        lib().orc_lc_free(h)
+    if n < 0:
+        raise RuntimeError("oracle binning failed (%d)" % n)
+    return out.reshape(k, cap)[:, :n].T.copy()
+
+
+def compute_img_aperture(rec, x, prec_angle, n_prec, ap_angle, n_app, nthreads=1):
+    """computeD2BI_img with apertureAngle and nAperturePoints: (My, Mx)."""
+    x, ld = _atoms_colmajor(x)
+    plan = D2biPlan()
+    if lib().orc_plan(C.byref(rec), len(x), float(prec_angle), int(n_prec), C.byref(plan)):
+        raise ValueError("orc_plan failed")
+    xy = _f64(aperture_points(n_app).reshape(-1))
+    img = np.zeros(plan.Mx * plan.My)
+    rc = lib().orc_compute_img_aperture(C.byref(rec), _p(x), ld, len(x), float(prec_angle), int(n_prec),
+                                        float(ap_angle), int(n_app), _p(xy), _p(img), int(nthreads))
+    if rc:
+        raise RuntimeError("oracle aperture image failed (%d)" % rc)
+    return img.reshape(plan.My, plan.Mx)
 
 
 def blur(img, t):

@@ -92,9 +92,9 @@ EXPORTED = [
     "d2bi_gpu_init", "d2bi_gpu_finalize", "d2bi_last_error", "d2bi_version",
     "d2bi_plan_image", "d2bi_setup_tilt_sg", "d2bi_setup_tilt_ur", "d2bi_setup_tilt_ng",
     "d2bi_rotate_to_standard_frame", "d2bi_wavelength", "d2bi_velocity",
-    "d2bi_deviation_parameter",
+    "d2bi_deviation_parameter", "d2bi_aperture_points", "d2bi_aperture_setup",
     "d2bi_gpu_atoms_upload", "d2bi_gpu_atoms_free", "d2bi_gpu_atoms_count",
-    "d2bi_gpu_image", "d2bi_gpu_image_dev", "d2bi_gpu_image_dev_out", "d2bi_gpu_image_batch", "d2bi_gpu_field",
+    "d2bi_gpu_image", "d2bi_gpu_image_aperture", "d2bi_gpu_image_dev", "d2bi_gpu_image_dev_out", "d2bi_gpu_image_batch", "d2bi_gpu_field",
     "d2bi_gpu_linkcells", "d2bi_gpu_lc_query", "d2bi_gpu_blur", "d2bi_synth_bcc",
 ]
 
@@ -133,12 +133,18 @@ def load():
     L.d2bi_velocity.argtypes = [C.c_double]
     L.d2bi_deviation_parameter.restype = C.c_double
     L.d2bi_deviation_parameter.argtypes = [C.c_double, _dp]
+    L.d2bi_aperture_points.argtypes = [C.c_int32, _dp]
+    L.d2bi_aperture_setup.argtypes = [C.POINTER(D2biRecord), C.c_double, C.c_int32, C.c_double,
+                                      C.c_int32, _dp, _dp]
     L.d2bi_gpu_atoms_upload.argtypes = [_dp, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]
     L.d2bi_gpu_atoms_free.argtypes = [C.c_void_p]
     L.d2bi_gpu_atoms_count.restype = C.c_int64
     L.d2bi_gpu_atoms_count.argtypes = [C.c_void_p]
     L.d2bi_gpu_image.argtypes = [C.POINTER(D2biRecord), _dp, C.c_int64, C.c_int64, C.c_double,
                                  C.c_int32, C.c_int32, _dp, C.POINTER(D2biStats)]
+    L.d2bi_gpu_image_aperture.argtypes = [C.POINTER(D2biRecord), _dp, C.c_int64, C.c_int64, C.c_double,
+                                          C.c_int32, C.c_double, C.c_int32, C.c_int32, _dp,
+                                          C.POINTER(D2biStats)]
     L.d2bi_gpu_image_dev.argtypes = [C.POINTER(D2biRecord), C.c_void_p, C.c_double, C.c_int32,
                                      C.c_int32, _dp, C.POINTER(D2biStats)]
     L.d2bi_gpu_image_dev_out.argtypes = [C.POINTER(D2biRecord), C.c_void_p, C.c_double, C.c_int32,

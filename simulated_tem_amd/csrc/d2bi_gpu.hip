@@ -974,6 +974,8 @@ struct IntParams {
     int Mx, My, Mz, nPrec;
     double b0, bg, dz;
     const double* aa;           // device array of 2 pi s_g per precession point
+    int accumulate;             // add the column sum to img instead of writing
+    double div;                 // written value: sum / div
 };
 
 // dynamicalTwoBeamIntegrationColumn_phaserho (Lib_DynamicalTwoBeamImaging.f90:2316-2480):
@@ -1043,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const doubl
         }
         acc = acc + (phigr * phigr + phigi * phigi);
     }
-    img[col] = acc / (1 * P.nPrec);
+    img[col] = P.accumulate ? img[col] + acc : acc / P.div;
 }
 
 // ------------------------------------------------------------------ (b) voxelise, FAST
@@ -1988,10 +1990,13 @@ static void launch_int_fast(const FastIntParams& P, bool dens, const void* field
 
 // FAST integration of all precession points, in passes of up to 32.
 // dens: the field holds float4 {x, rho} (density field on), else float2 x.
-static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, double* d_img, hipStream_t s)
+// acc_in: add to d_img from the first pass on; inv: weight of one
+// integration (0: 1/nPrec).
+static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, double* d_img, hipStream_t s,
+                                 bool acc_in = false, double inv = 0.0)
 {
-    const double inv = 1.0 / p->nPrec;
-    for (int p0 = 0, first = 1; p0 < p->nPrec; first = 0) {
+    if (inv == 0.0) inv = 1.0 / p->nPrec;
+    for (int p0 = 0, first = acc_in ? 0 : 1; p0 < p->nPrec; first = 0) {
         const int left = p->nPrec - p0;
         const int np = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
         FastIntParams P;
@@ -2012,11 +2017,15 @@ static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, d
     return D2BI_OK;
 }
 
-static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double* d_img, hipStream_t s)
+// accumulate: add each column's sum over precession points to d_img;
+// otherwise write sum / nPrec (div = 0) or sum / div.
+static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double* d_img, hipStream_t s,
+                                 int accumulate = 0, double div = 0.0)
 {
     Workspace& w = g.ws;
     const int64_t plane = (int64_t)p->Mx * p->My;
-    IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa};
+    IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa, accumulate,
+                div == 0.0 ? (double)p->nPrec : div};
     hipLaunchKernelGGL(k_integrate_fp64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
                        w.f64_re.as<double>(), w.f64_im.as<double>(), w.f64_rho.as<double>(), d_img);
     HIPCHK(hipGetLastError());
@@ -2087,6 +2096,82 @@ static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, d
         st->ms_voxel += T.ms(2, 3);          // the voxel kernel alone
         st->ms_integrate += T.ms(4, 5);      // the integration kernels alone
         st->ms_total += T.ms(0, 6);
+    }
+    if (plan_out) *plan_out = plan;
+    return D2BI_OK;
+}
+
+__global__ void k_div_f64(double* __restrict__ p, int64_t n, double d)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = p[i] / d;
+}
+
+// computeD2BI_img with apertureAngle and nAperturePoints > 1
+// (Lib_DynamicalTwoBeamImaging.f90:1775-1786, 1853-1866, 1905-1951).  Every
+// aperture point has its own g-vector for g.r (findGdotR_aperture :1269-1396)
+// and so its own phase field; the link cells, ranges and densities are those
+// of the unshifted condition.  Each point is binned, voxelised and integrated
+// at its nPrec deviation parameters aa(app, :), the column sums are added up
+// and the image is divided by nApp * nPrec at the end (the reference adds
+// over (prec, app) in the other nesting; FP64 sums agree to rounding).
+// Result left in ws.img.
+static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
+                          double apAngle, int nApp, int mode, d2bi_stats* st, d2bi_plan* plan_out)
+{
+    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
+    d2bi_plan plan;
+    CHK(plan_image(r, n, precAngle, nPrec, &plan));
+    std::vector<double> gshift(3 * (size_t)nApp), aa((size_t)nApp * nPrec);
+    CHK(aperture_setup(r, precAngle, nPrec, apAngle, nApp, gshift.data(), aa.data()));
+    AxisTables tab;
+    CHK(axis_tables(r, &plan, &tab));
+    hipStream_t s = g.stream;
+    AxisDev ax;
+    CHK(upload_axis(tab, &ax, s));
+    Workspace& w = g.ws;
+    const int64_t plane = (int64_t)plan.Mx * plan.My;
+    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
+    double* d_img = w.img.as<double>();
+    const bool dens = r->use_density != 0;
+    const double total = (double)nApp * nPrec;
+    for (int app = 0; app < nApp; app++) {
+        d2bi_plan pk = plan;
+        for (int i = 0; i < 3; i++) pk.gg[i] = gshift[3 * app + i];
+        for (int q = 0; q < nPrec; q++) pk.aa[q] = aa[app + (size_t)nApp * q];
+        BinParams BP;
+        CHK(bin_params(r, &pk, &BP));
+        Binned b;
+        CHK(run_binning(d_x, n, BP, &b, s));
+        CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s));
+        if (mode == D2BI_MODE_FP64) {
+            double* d_aa = d_img + plane;
+            HIPCHK(hipMemcpyAsync(d_aa, pk.aa, sizeof(double) * nPrec, hipMemcpyHostToDevice, s));
+            // app 0 writes its sum (div 1), the others add theirs
+            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0));
+        } else {
+            CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total));
+        }
+        int64_t* hc = nullptr;
+        if (st) {
+            void* hp;
+            CHK(pinned(16, &hp));
+            hc = (int64_t*)hp;
+            HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
+        }
+        // pk.aa (host) is read by the copy above: finish before the next point
+        HIPCHK(hipStreamSynchronize(s));
+        if (st) {
+            st->n_points += b.nr;
+            st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
+            st->max_per_cell = std::max(st->max_per_cell, (int)(hc[0] & 0xffffffff));
+            st->n_cells = b.ncell;
+        }
+    }
+    if (mode == D2BI_MODE_FP64) {
+        hipLaunchKernelGGL(k_div_f64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, d_img, plane, total);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
     }
     if (plan_out) *plan_out = plan;
     return D2BI_OK;
@@ -2520,6 +2605,22 @@ int d2bi_gpu_image_dev_out(const d2bi_record* rec, d2bi_atoms atoms, double prec
     CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
     HIPCHK(hipMemcpyAsync(d_img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_image_aperture(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms,
+                            double precAngle, int32_t nPrec, double apertureAngle, int32_t nAperture,
+                            int32_t mode, double* img, d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (!rec || !img) return seterr(D2BI_ERR_ARG, "null argument");
+    if (nAperture < 1 || nAperture > D2BI_MAX_APERTURE) return seterr(D2BI_ERR_ARG, "nAperture out of range");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    double* d_x;
+    CHK(upload_atoms_tmp(x, ld, nAtoms, &d_x));
+    d2bi_plan plan;
+    CHK(image_aperture(rec, d_x, nAtoms, precAngle, nPrec, apertureAngle, nAperture, mode, stats, &plan));
+    HIPCHK(hipMemcpy(img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToHost));
     return D2BI_OK;
 }
 

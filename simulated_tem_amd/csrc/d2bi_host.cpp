@@ -182,6 +182,24 @@ int estimate_max_count(double nCells, int nTotal)
     return nTotal;
 }
 
+// The precession-tilted g-vector of point prec of nPrec (computeD2BI_img
+// :1805-1822): prec = 1 is the untilted getAperture; otherwise the Rodrigues
+// rotation by the precession angle about (sin phi, cos phi, 0),
+// phi = (prec - 1) 2 pi / (nPrec - 1).
+static void precession_g(const d2bi_record* r, double costheta, double sintheta, int prec, int nPrec, double* gg)
+{
+    if (prec == 1) {
+        aperture(r, nullptr, gg);
+        return;
+    }
+    double phi = (prec - 1) * 2 * PI / (nPrec - 1);
+    double ux = std::sin(phi), uy = std::cos(phi), T[9];
+    m3(T,1,1) = costheta + ux * ux * (1 - costheta); m3(T,1,2) = ux * uy * (1 - costheta);             m3(T,1,3) = uy * sintheta;
+    m3(T,2,1) = ux * uy * (1 - costheta);             m3(T,2,2) = costheta + uy * uy * (1 - costheta); m3(T,2,3) = -ux * sintheta;
+    m3(T,3,1) = -uy * sintheta;                       m3(T,3,2) = ux * sintheta;                       m3(T,3,3) = costheta;
+    aperture(r, T, gg);
+}
+
 // computeD2BI_img prologue (Lib_DynamicalTwoBeamImaging.f90:1635-1836) for
 // one rank: factoriseParallel(Mx,My,1) gives one patch (Lib_FactoriseParallel.f90:104),
 // nAperturePoints = 1 gives a zero aperture offset.
@@ -231,21 +249,107 @@ int plan_image(const d2bi_record* r, int64_t nAtoms, double precAngle, int nPrec
     p->b0 = PI / r->xi0;
     for (int prec = 1; prec <= nPrec; prec++) {
         double gg[3];
-        if (prec == 1) {
-            aperture(r, nullptr, gg);
-        } else {
-            // Rodrigues rotation by the precession angle about (sin phi, cos phi, 0) (:1814-1820)
-            double phi = (prec - 1) * 2 * PI / (nPrec - 1);
-            double ux = std::sin(phi), uy = std::cos(phi), T[9];
-            m3(T,1,1) = costheta + ux * ux * (1 - costheta); m3(T,1,2) = ux * uy * (1 - costheta);             m3(T,1,3) = uy * sintheta;
-            m3(T,2,1) = ux * uy * (1 - costheta);             m3(T,2,2) = costheta + uy * uy * (1 - costheta); m3(T,2,3) = -ux * sintheta;
-            m3(T,3,1) = -uy * sintheta;                       m3(T,3,2) = ux * sintheta;                       m3(T,3,3) = costheta;
-            aperture(r, T, gg);
-        }
+        precession_g(r, costheta, sintheta, prec, nPrec, gg);
         double g2[3] = {gg[0] + 0.0, gg[1] + 0.0, gg[2] + 0.0};
         p->aa[prec - 1] = 2 * PI * deviation_parameter(r->E, g2);
     }
     p->rc2 = (3 * p->sigma) * (3 * p->sigma);
+    return D2BI_OK;
+}
+
+// The arrangements of n = 1..25 points in the unit disk that
+// evenlySpacedPointsInCircle returns from its table (src/evenlySpaced.h,
+// xKnown): the same values, written as the default-real (FP32) literals they
+// are there; set n holds its n points (x, y) and starts at n (n - 1).
+static const int APERTURE_KNOWN = 25;
+static const float APERTURE_XY[APERTURE_KNOWN * (APERTURE_KNOWN + 1)] = {
+    /* n = 1 */ 0.0000f, 0.0000f,
+    /* n = 2 */ 0.0033f, 0.4139f, 0.0013f, -0.4234f,
+    /* n = 3 */ 0.2856f, -0.4698f, 0.2730f, 0.4802f, -0.5436f, -0.0069f,
+    /* n = 4 */ -0.4189f, 0.4201f, 0.4250f, -0.4263f, -0.4162f, -0.4287f, 0.4276f, 0.4157f,
+    /* n = 5 */ 0.0000f, 0.6230f, 0.5925f, 0.1925f, -0.5925f, 0.1925f, 0.3662f, -0.5040f, -0.3662f, -0.5040f,
+    /* n = 6 */ 0.0962f, 0.6590f, -0.0001f, -0.0001f, -0.4775f, -0.4790f, 0.6566f, 0.1037f, -0.5890f, 0.2967f, 0.2995f, -0.5873f,
+    /* n = 7 */ 0.3594f, 0.6018f, -0.6860f, -0.0086f, 0.3376f, -0.6123f, -0.3477f, 0.5971f, -0.3484f, -0.5998f, 0.0086f, -0.0080f, 0.7051f, -0.0060f,
+    /* n = 8 */ 0.6889f, -0.1686f, 0.0038f, 0.7079f, -0.3111f, -0.6400f, 0.0016f, 0.0005f, 0.5710f, 0.4348f, 0.3032f, -0.6378f, -0.5613f, 0.4424f, -0.6888f, -0.1576f,
+    /* n = 9 */ 0.6646f, 0.2658f, 0.2842f, 0.6581f, -0.6610f, 0.2900f, -0.6678f, -0.2635f, -0.2833f, -0.6631f, 0.6514f, -0.2895f, -0.0023f, 0.0005f, -0.2673f, 0.6766f, 0.2629f, -0.6723f,
+    /* n = 10 */ 0.2373f, -0.6995f, -0.1931f, 0.7037f, 0.6264f, -0.2817f, -0.6888f, -0.2767f, 0.1319f, 0.2148f, -0.1001f, -0.2027f, -0.6039f, 0.2969f, -0.3247f, -0.6950f, 0.3586f, 0.6971f, 0.7013f, 0.2768f,
+    /* n = 11 */ 0.3003f, 0.6941f, 0.6673f, -0.2784f, 0.7005f, 0.2817f, 0.1181f, -0.2787f, 0.1747f, 0.2016f, -0.2704f, 0.6578f, 0.3067f, -0.7240f, -0.2731f, 0.0124f, -0.7039f, 0.2876f, -0.2797f, -0.6767f, -0.6963f, -0.2989f,
+    /* n = 12 */ -0.3180f, -0.6993f, 0.7739f, 0.1049f, -0.4182f, 0.6592f, -0.7286f, 0.2559f, 0.6023f, -0.4005f, 0.0807f, 0.7225f, -0.6525f, -0.2809f, -0.2132f, 0.1957f, 0.2107f, -0.7415f, 0.2788f, 0.1516f, 0.5652f, 0.5639f, -0.0172f, -0.2730f,
+    /* n = 13 */ -0.1471f, -0.7076f, 0.7065f, -0.3687f, 0.3496f, -0.7076f, -0.2945f, -0.0995f, 0.0604f, 0.3004f, 0.5327f, 0.5633f, 0.7069f, 0.1266f, 0.1402f, 0.7875f, -0.7937f, -0.0975f, -0.5862f, -0.5237f, 0.2236f, -0.2110f, -0.3006f, 0.7006f, -0.6406f, 0.3619f,
+    /* n = 14 */ -0.7275f, -0.1520f, -0.5471f, -0.5864f, 0.7013f, -0.3425f, -0.2605f, 0.1259f, -0.3260f, 0.6751f, 0.1584f, 0.3358f, -0.1551f, -0.3397f, 0.2597f, -0.1274f, 0.5492f, 0.5868f, -0.1374f, -0.7951f, 0.7228f, 0.1550f, 0.3247f, -0.6801f, 0.1308f, 0.7898f, -0.7022f, 0.3449f,
+    /* n = 15 */ -0.1422f, -0.2654f, 0.5549f, 0.5720f, 0.3199f, -0.1904f, -0.3215f, 0.1717f, 0.5268f, -0.6078f, -0.6725f, -0.3334f, -0.7810f, 0.1009f, 0.7698f, -0.2567f, -0.2621f, 0.7040f, 0.1703f, 0.7755f, 0.7312f, 0.1747f, 0.1118f, -0.7450f, -0.6463f, 0.5109f, 0.1538f, 0.2725f, -0.3652f, -0.7051f,
+    /* n = 16 */ -0.1425f, -0.8263f, 0.3192f, 0.7491f, -0.3469f, -0.1454f, -0.0851f, -0.4567f, -0.5411f, 0.5850f, -0.7588f, -0.2732f, 0.6504f, -0.4274f, 0.7421f, 0.0413f, 0.6535f, 0.4786f, 0.2439f, 0.3046f, -0.2068f, 0.2518f, -0.7447f, 0.1788f, -0.1177f, 0.7505f, -0.5076f, -0.6124f, 0.2879f, -0.7184f, 0.2160f, -0.1549f,
+    /* n = 17 */ 0.3161f, -0.6961f, 0.7604f, 0.0421f, 0.3137f, -0.2210f, -0.1790f, 0.8009f, 0.0068f, 0.0087f, -0.5505f, 0.5547f, -0.3949f, 0.0686f, -0.1523f, -0.3575f, -0.7933f, 0.1922f, -0.0899f, 0.3987f, 0.6943f, -0.4170f, 0.3265f, 0.2362f, -0.7301f, -0.2495f, 0.6556f, 0.4892f, -0.5217f, -0.6202f, 0.2699f, 0.7318f, -0.1202f, -0.7983f,
+    /* n = 18 */ -0.7911f, -0.2109f, 0.0114f, 0.0026f, -0.5840f, -0.5665f, 0.4079f, -0.0799f, 0.5603f, -0.5419f, -0.2173f, -0.7520f, 0.2176f, 0.7831f, -0.5727f, 0.5910f, -0.2718f, 0.2916f, 0.0768f, -0.3942f, 0.7977f, -0.1884f, 0.2155f, -0.7916f, 0.2083f, 0.3514f, 0.5645f, 0.5914f, -0.1812f, 0.7497f, 0.7394f, 0.2412f, -0.7427f, 0.1961f, -0.3600f, -0.1834f,
+    /* n = 19 */ 0.5823f, 0.5849f, -0.2985f, -0.3013f, 0.5655f, -0.5519f, 0.4129f, -0.0942f, -0.5733f, -0.5940f, -0.5599f, 0.5592f, -0.1124f, 0.4167f, -0.7626f, -0.2152f, -0.4058f, 0.1159f, 0.2056f, 0.7598f, 0.7620f, 0.2131f, 0.7966f, -0.2076f, -0.2079f, 0.7953f, -0.7934f, 0.2078f, 0.3019f, 0.3115f, 0.0020f, 0.0068f, 0.1144f, -0.4037f, 0.2255f, -0.7911f, -0.1946f, -0.7611f,
+    /* n = 20 */ 0.4424f, 0.7036f, -0.2951f, 0.7814f, -0.0001f, 0.0001f, 0.5143f, -0.6032f, 0.4078f, -0.1321f, -0.8061f, 0.2028f, -0.2948f, -0.2994f, 0.6825f, 0.4634f, 0.3019f, 0.3235f, 0.7812f, 0.1268f, 0.0898f, 0.7776f, -0.5925f, 0.5199f, 0.7769f, -0.2838f, -0.5867f, -0.5830f, 0.1010f, -0.4216f, -0.2134f, -0.7668f, -0.1463f, 0.4002f, -0.7656f, -0.2059f, -0.4318f, 0.1006f, 0.1718f, -0.8091f,
+    /* n = 21 */ 0.8165f, -0.1460f, 0.5352f, 0.6043f, -0.8184f, -0.1478f, 0.7606f, 0.2509f, -0.4306f, -0.0769f, 0.0159f, -0.7999f, -0.3625f, -0.7562f, 0.3833f, -0.7482f, -0.6344f, -0.4840f, 0.6463f, -0.4836f, -0.2084f, 0.7760f, 0.0104f, 0.0054f, -0.5717f, 0.6030f, 0.3575f, 0.2481f, 0.0662f, 0.4520f, 0.2036f, -0.4038f, -0.7528f, 0.2409f, -0.1885f, -0.4090f, -0.2975f, 0.3129f, 0.4449f, -0.1051f, 0.1974f, 0.8049f,
+    /* n = 22 */ 0.0085f, 0.4460f, -0.0187f, -0.7993f, 0.0108f, 0.8365f, -0.6532f, 0.5300f, -0.2132f, -0.3979f, 0.1917f, -0.4111f, 0.4432f, -0.1274f, -0.6452f, -0.4921f, -0.3593f, 0.2846f, 0.8167f, -0.2173f, 0.3603f, 0.7114f, -0.3781f, -0.7477f, 0.3468f, -0.7635f, 0.6209f, -0.5145f, -0.7900f, 0.2032f, -0.3486f, 0.7302f, 0.0012f, -0.0004f, 0.3639f, 0.2640f, -0.8332f, -0.1632f, 0.6752f, 0.5002f, -0.4575f, -0.0914f, 0.7897f, 0.1464f,
+    /* n = 23 */ -0.4765f, -0.6921f, -0.6688f, 0.4623f, 0.7500f, -0.3959f, -0.3329f, -0.3293f, -0.1610f, -0.7879f, 0.5511f, 0.5885f, -0.2581f, 0.3898f, 0.2623f, 0.8028f, -0.0002f, -0.0008f, -0.8257f, 0.1656f, -0.4415f, 0.7195f, 0.4960f, -0.6411f, 0.0725f, -0.4485f, -0.6994f, -0.4664f, 0.1960f, -0.8234f, -0.7959f, -0.1691f, 0.4021f, -0.2222f, -0.4644f, 0.0538f, 0.4364f, 0.1729f, -0.0950f, 0.7977f, 0.7973f, -0.0551f, 0.7866f, 0.3060f, 0.1419f, 0.4365f,
+    /* n = 24 */ 0.6013f, -0.5800f, 0.8014f, 0.0627f, 0.3947f, 0.1186f, -0.1415f, 0.4891f, 0.2353f, -0.5040f, 0.4815f, -0.2330f, -0.2643f, 0.8082f, -0.3289f, -0.1665f, 0.4505f, 0.7163f, 0.6824f, 0.4334f, -0.5605f, 0.5859f, -0.4159f, 0.2031f, -0.7863f, 0.2494f, 0.0984f, -0.1833f, -0.5930f, -0.5287f, 0.3545f, -0.7979f, 0.0269f, -0.8269f, -0.3280f, -0.7932f, 0.0994f, 0.8117f, 0.8023f, -0.3052f, -0.0135f, 0.1498f, -0.7804f, -0.1618f, -0.1447f, -0.4981f, 0.2362f, 0.4391f,
+    /* n = 25 */ 0.8145f, 0.1926f, 0.4560f, 0.6887f, 0.6773f, -0.4332f, 0.1368f, -0.8009f, 0.4587f, -0.6934f, 0.1329f, 0.4714f, 0.4648f, 0.3014f, 0.2515f, -0.3455f, -0.8148f, -0.1139f, 0.8440f, -0.1374f, -0.4449f, -0.2826f, -0.7389f, -0.4359f, 0.1303f, 0.8328f, -0.2030f, -0.8254f, 0.7119f, 0.4914f, -0.2229f, 0.7635f, -0.5702f, 0.5641f, 0.1746f, 0.0914f, -0.4918f, -0.6601f, -0.1979f, 0.3184f, -0.1157f, -0.1161f, -0.1193f, -0.4867f, -0.4816f, 0.0866f, 0.5036f, -0.0519f, -0.8046f, 0.2541f,
+};
+
+// evenlySpacedPointsInCircle(x) without a file
+// (Lib_EvenlySpacedPointsInCircle.f90:29-100): n <= 25 points from the table,
+// more from the sunflower placement with alpha = 0 (:438-463) and radius
+// (:465-478), in that module's PI = 3.141592654.  xy is (2, n), x first.
+int aperture_points(int n, double* xy)
+{
+    if (n < 1) return seterr(D2BI_ERR_ARG, "aperture points must be >= 1");
+    if (n <= APERTURE_KNOWN) {
+        const float* t = APERTURE_XY + n * (n - 1);
+        for (int k = 0; k < 2 * n; k++) xy[k] = (double)t[k];
+        return D2BI_OK;
+    }
+    const double pi = 3.141592654;
+    const double phi = (std::sqrt(5.0) + 1.0) / 2;
+    const double alpha = 0.0;
+    const int bb = (int)std::lround(std::sqrt(alpha * alpha * n));
+    for (int kk = 1; kk <= n; kk++) {
+        double rr = 1;
+        if (kk <= n - bb) rr = std::sqrt((double)(2 * kk - 1) / (2 * n - bb - 1));
+        const double theta = 2 * pi * kk / (phi * phi);
+        xy[2 * (kk - 1)] = rr * std::cos(theta);
+        xy[2 * (kk - 1) + 1] = rr * std::sin(theta);
+    }
+    return D2BI_OK;
+}
+
+// computeD2BI_img with apertureAngle and nAperturePoints
+// (Lib_DynamicalTwoBeamImaging.f90:1655-1667, 1805-1836) and
+// findGdotR_aperture (:1336-1344): the aperture points scaled to
+// modg = apertureAngle 2 pi / lambda, the shifted g-vector of each point for
+// g.r (the untilted g plus the point's x and y), and
+// aa(app, prec) = 2 pi s_g(g_prec + dg_app), app fastest.
+int aperture_setup(const d2bi_record* r, double precAngle, int nPrec, double apertureAngle, int nApp,
+                   double* gshift, double* aa)
+{
+    if (nPrec < 1 || nPrec > D2BI_MAX_PREC) return seterr(D2BI_ERR_ARG, "nPrec out of range");
+    if (nApp < 1 || nApp > D2BI_MAX_APERTURE) return seterr(D2BI_ERR_ARG, "nAperture out of range");
+    const double modk = 2 * PI / wavelength(r->E * 1000);
+    const double modg = apertureAngle * modk;
+    std::vector<double> xy(2 * nApp);
+    HCHK(aperture_points(nApp, xy.data()));
+    double gg0[3];
+    aperture(r, nullptr, gg0);
+    for (int app = 0; app < nApp; app++) {
+        const double dg[3] = {modg * xy[2 * app], modg * xy[2 * app + 1], modg * 0.0};
+        gshift[3 * app] = gg0[0] + dg[0];
+        gshift[3 * app + 1] = gg0[1] + dg[1];
+        gshift[3 * app + 2] = gg0[2];
+    }
+    double costheta = 1.0, sintheta = 0.0;
+    if (nPrec > 1) { costheta = std::cos(precAngle); sintheta = std::sin(precAngle); }
+    for (int prec = 1; prec <= nPrec; prec++) {
+        double gg[3];
+        precession_g(r, costheta, sintheta, prec, nPrec, gg);
+        for (int app = 0; app < nApp; app++) {
+            const double dg[3] = {modg * xy[2 * app], modg * xy[2 * app + 1], modg * 0.0};
+            double g2[3] = {gg[0] + dg[0], gg[1] + dg[1], gg[2] + dg[2]};
+            aa[app + nApp * (prec - 1)] = 2 * PI * deviation_parameter(r->E, g2);
+        }
+    }
     return D2BI_OK;
 }
 
@@ -1105,6 +1209,20 @@ int d2bi_rotate_to_standard_frame(const double k[3], const double a_cell[9], con
     if (!k || !a_cell || !hkl || !U) return d2bi::seterr(D2BI_ERR_ARG, "null argument");
     d2bi::rotate_to_standard_frame(k, a_cell, hkl, U);
     return D2BI_OK;
+}
+
+int d2bi_aperture_points(int32_t n, double* xy)
+{
+    if (!xy) return d2bi::seterr(D2BI_ERR_ARG, "null argument");
+    if (n > D2BI_MAX_APERTURE) return d2bi::seterr(D2BI_ERR_ARG, "nAperture out of range");
+    return d2bi::aperture_points(n, xy);
+}
+
+int d2bi_aperture_setup(const d2bi_record* rec, double precAngle, int32_t nPrec, double apertureAngle,
+                        int32_t nAperture, double* gshift, double* aa)
+{
+    if (!rec || !gshift || !aa) return d2bi::seterr(D2BI_ERR_ARG, "null argument");
+    return d2bi::aperture_setup(rec, precAngle, nPrec, apertureAngle, nAperture, gshift, aa);
 }
 
 double d2bi_wavelength(double V) { return d2bi::wavelength(V); }

@@ -17,6 +17,9 @@ double velocity(double V);
 double deviation_parameter(double E_keV, const double* g);
 int estimate_max_count(double nCells, int nTotal);
 int plan_image(const d2bi_record* r, int64_t nAtoms, double precAngle, int nPrec, d2bi_plan* p);
+int aperture_points(int n, double* xy);
+int aperture_setup(const d2bi_record* r, double precAngle, int nPrec, double apertureAngle, int nApp,
+                   double* gshift, double* aa);
 void rotate_to_standard_frame(const double* z0, const double* A, const double* hkl, double* U);
 int setup_tilt_sg(d2bi_record* r, const double* k, double sg, int mxy, int mzz, const double* d);
 int setup_tilt_ur(d2bi_record* r, const double* U, const double* R, int mxy, int mzz, const double* d);

@@ -216,6 +216,17 @@ class DynamicalTwoBeamImaging:
                                          int(n_prec), C.byref(p)))
         return p
 
+    def aperture(self, precessionAngle, nPrecessionAngle, apertureAngle, nAperturePoints):
+        """(gshift (nApp,3), aa (nPrec, nApp)): the g-vector for g.r at each
+        aperture point and 2 pi s_g per (precession point, aperture point)
+        (Lib_DynamicalTwoBeamImaging.f90:1655-1667, 1805-1836)."""
+        gs = np.zeros(3 * nAperturePoints)
+        aa = np.zeros(nAperturePoints * nPrecessionAngle)
+        check(abi.load().d2bi_aperture_setup(C.byref(self.rec), float(precessionAngle),
+                                             int(nPrecessionAngle), float(apertureAngle),
+                                             int(nAperturePoints), ptr(gs), ptr(aa)))
+        return gs.reshape(nAperturePoints, 3), aa.reshape(nPrecessionAngle, nAperturePoints)
+
     def findImageParameters(self):
         """(aperpx, xmax[3], modg, Mx, My, Mz) (:450-482)."""
         p = self.plan(1)
@@ -223,15 +234,29 @@ class DynamicalTwoBeamImaging:
 
     # --- the hot path
     def computeD2BI_img(self, x, precessionAngle=0.0, nPrecessionAngle=1, mode="fast",
-                        stats=None, out=None):
+                        stats=None, out=None, apertureAngle=None, nAperturePoints=None):
         """Image of one condition, (My, Mx) array.  x: (nAtoms,3) array or Atoms.
         out: optional DeviceBuffer of shape (My, Mx) that receives the image
-        in HBM (x must then be Atoms); it is returned."""
+        in HBM (x must then be Atoms); it is returned.
+        apertureAngle (radians) and nAperturePoints are computeD2BI_img's
+        optional aperture sampling (Lib_DynamicalTwoBeamImaging.f90:1563,
+        1655-1667): the image is then the mean over the aperture points,
+        each with its own g.r (x must be an (nAtoms,3) array)."""
         _ensure()
         L = abi.load()
         p = self.plan(1)
         st = stats if stats is not None else D2biStats()
         m = MODES[mode]
+        if nAperturePoints is not None:
+            if out is not None or isinstance(x, Atoms):
+                raise TypeError("aperture sampling takes host atom arrays and returns a host image")
+            img = np.zeros((p.My, p.Mx))
+            x = f64(x)
+            check(L.d2bi_gpu_image_aperture(C.byref(self.rec), ptr(x), 3, len(x), float(precessionAngle),
+                                            int(nPrecessionAngle),
+                                            float(apertureAngle if apertureAngle is not None else 0.0),
+                                            int(nAperturePoints), m, ptr(img), C.byref(st)))
+            return img
         if out is not None:
             if not isinstance(x, Atoms):
                 raise TypeError("a device output needs device-resident Atoms")
@@ -351,6 +376,15 @@ def gaussianBlurImage(f_in, t):
     out = np.zeros_like(f)
     check(abi.load().d2bi_gpu_blur(ptr(f), nx, ny, float(t), ptr(out)))
     return out
+
+
+def evenlySpacedPointsInCircle(n):
+    """n points in the unit disk, (n, 2): the reference's known arrangements
+    for n <= 25, the sunflower placement beyond
+    (Lib_EvenlySpacedPointsInCircle.f90:29-100)."""
+    xy = np.zeros(2 * int(n))
+    check(abi.load().d2bi_aperture_points(int(n), ptr(xy)))
+    return xy.reshape(int(n), 2)
 
 
 def wavelength(V):

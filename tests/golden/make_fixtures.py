@@ -10,6 +10,9 @@ Writes, next to this script:
   w_loop_golden.npz     Data/test/test.i111_loop_W.lammps atom positions and
                         box, and the golden image + header of
                         Data/test/...g2_0_0.k0.00_0.00_1.00_ng1.00.dat
+  aperture_points.json  the table of n = 1..25 points in the unit disk that
+                        evenlySpacedPointsInCircle returns (src/evenlySpaced.h),
+                        as the decimal literals written there
 The fixtures are data (inputs and expected outputs); no reference source
 text is stored.
 """
@@ -96,12 +99,32 @@ def w_loop(ref):
         header="\n".join(dl[:11]))
 
 
+def aperture_points(ref):
+    """xKnown(2, 25, 25) of src/evenlySpaced.h: the literal list fills x(1:2,
+    point, n) first index fastest; point k of the n-point set for k <= n.
+    The literals have no kind suffix (default real), so they are kept as the
+    decimal strings written there."""
+    src = open(os.path.join(ref, "src", "evenlySpaced.h")).read()
+    maxk = int(re.search(r"MAXKNOWN\s*=\s*(\d+)", src).group(1))
+    m = re.search(r"reshape\(\s*\(/(.*?)/\)\s*,\s*\(/\s*2\s*,\s*MAXKNOWN\s*,\s*MAXKNOWN\s*/\)", src, re.S)
+    lits = re.findall(r"[-+]?\d+\.\d+", m.group(1))
+    assert len(lits) == 2 * maxk * maxk, len(lits)
+    pts = {}
+    for n in range(1, maxk + 1):
+        base = 2 * maxk * (n - 1)
+        pts[str(n)] = [[lits[base + 2 * k], lits[base + 2 * k + 1]] for k in range(n)]
+    return {"source": "src/evenlySpaced.h (xKnown, MAXKNOWN = %d)" % maxk,
+            "kind": "default real (FP32) literals", "points": pts}
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     with open(os.path.join(HERE, "linkcell_kat.json"), "w") as f:
         json.dump(linkcell_kat(ref), f, indent=1)
     w = w_loop(ref)
     np.savez_compressed(os.path.join(HERE, "w_loop_golden.npz"), **w)
+    with open(os.path.join(HERE, "aperture_points.json"), "w") as f:
+        json.dump(aperture_points(ref), f, indent=0)
     print("fixtures written to", HERE)
 
 

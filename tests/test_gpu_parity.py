@@ -140,6 +140,28 @@ def test_image_fp64_and_fast(case, n_prec, fast_variant):
     assert ef.max() <= FAST_TOL
 
 
+@pytest.mark.parametrize("n_app,ap_angle,density", [(1, 1e-4, False), (8, 1e-3, True), (30, 3e-3, False)])
+def test_image_aperture(n_app, ap_angle, density, fast_variant):
+    """computeD2BI_img with apertureAngle / nAperturePoints against the
+    oracle's joint voxel loop (one phase sum per aperture point, integrations
+    in (prec, app) order); nApp = 1 is the plain image bit for bit.  n = 8
+    takes the reference's table of points, n = 30 the sunflower placement."""
+    x, im = _cond("c0", cells=(9, 11, 7), g=(1, -1, 0), k=(1, 1, 1), d=(25.0, 27.0, 19.0), sg=0.01,
+                  density=density)
+    ref = orc.compute_img_aperture(im.rec, x, 0.005, 3, ap_angle, n_app, nthreads=NTH)
+    g64 = im.computeD2BI_img(x, 0.005, 3, mode="fp64", apertureAngle=ap_angle, nAperturePoints=n_app)
+    gf = im.computeD2BI_img(x, 0.005, 3, mode="fast", apertureAngle=ap_angle, nAperturePoints=n_app)
+    e64 = np.max(np.abs(g64 - ref))
+    ef = np.abs(gf - ref)
+    print("[aperture] n=%d angle=%g v%s fp64 max %.3e  fast max %.3e mean %.3e"
+          % (n_app, ap_angle, fast_variant, e64, ef.max(), ef.mean()), flush=True)
+    assert e64 <= FP64_TOL
+    assert ef.max() <= FAST_TOL
+    if n_app == 1:
+        assert np.array_equal(g64, im.computeD2BI_img(x, 0.005, 3, mode="fp64"))
+        assert np.array_equal(gf, im.computeD2BI_img(x, 0.005, 3, mode="fast"))
+
+
 def test_blur_matches_oracle():
     rng = np.random.default_rng(7)
     for (ny, nx, t) in [(94, 94, 3.1446), (800, 800, 3.16), (33, 517, 0.4), (300, 7, 12.3)]:
