@@ -1055,6 +1055,21 @@ static constexpr int GUARD_SLOTS = 1024;             // guard counters (power of
 static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart
 static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
 
+// Inclusive prefix sum over the 64 lanes of a wave in six DPP steps: within
+// each row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill), then row 0's
+// total into row 1 and row 2's into row 3 (row_bcast:15), then rows 0-1's
+// into rows 2-3 (row_bcast:31).  All lanes must be active.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
 struct FastParams {
     VoxParams v;
     int ntx, nty, ntz;          // tiles per axis
@@ -1188,7 +1203,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     // (-2 k2 p_y, -2 k2 p_z, k2 (p_y^2 + p_z^2), cos), then (sin, slot, cell
     // offsets, 0), and four zero records after the last candidate of a pass
     constexpr int REC = ROW ? 3 : 2;
-    constexpr int CAP = ROW ? 128 : TILE_CAP;            // staged candidates per pass
+    // ROW: 96 staged candidates per pass keep the wave's LDS at 7.6 KB, so
+    // its 88 VGPRs, not LDS, bound occupancy (20 waves per CU; 128 allowed 17
+    // and ran 15 % slower on c1)
+    constexpr int CAP = ROW ? 96 : TILE_CAP;
     constexpr int NREC = ROW ? CAP + 4 : CAP;
     __shared__ float4 crec[TILE_MAXC];         // kept cells: origin in the tile frame, slot - index
     __shared__ int cupk[TILE_MAXC];            // kept cells: dx | dy << 8 | dz << 16
@@ -1204,7 +1222,9 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     // kept cells: in range of the tile box and not empty, in cell order
     int K = 0, T = 0;
     {
-        const float inx = 1.0f / (float)ncx, inxy = 1.0f / (float)(ncx * ncy);
+        // (c + 0.5) / n lies at least 1/(2n) >= 1/256 from an integer, far
+        // beyond the error of v_rcp_f32, so the truncations below are exact
+        const float inx = __builtin_amdgcn_rcpf((float)ncx), inxy = __builtin_amdgcn_rcpf((float)(ncx * ncy));
         for (int cb = 0; cb < ncell; cb += 64) {
             const int c = cb + lane;
             int cnt = 0, b = 0, upk = 0;
@@ -1232,12 +1252,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     upk = dx | (dy << 8) | (dz << 16);
                 }
             }
-            int incl = cnt;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += t;
-            }
+            const int incl = wave_incl_scan(cnt);
             const unsigned long long m = __ballot(cnt > 0);
             if (cnt > 0) {
                 const int k = K + (int)__popcll(m & lt);
