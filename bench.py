@@ -36,7 +36,7 @@ TRANS_PER_VOXEL = 30.0
 # passes scaled by a 1 GiB copy's counts (profiles/k_voxel_tile_c1_traffic.md);
 # a profiler cannot run inside the timed process, so the figure is recorded
 # per workload
-TRAFFIC_BYTES = {"c1": 2.950e9}
+TRAFFIC_BYTES = {"c1": 2.963e9}
 
 
 def log(*a):

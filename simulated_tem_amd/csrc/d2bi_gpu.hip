@@ -1216,8 +1216,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     static_assert(sizeof(int) * TILE_MAXC <= sizeof(float4) * REC * NREC, "cstart overlay");
     static_assert(!ROW || 3 * 256 * sizeof(float) <= sizeof(float4) * REC * NREC, "row sums overlay");
 
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const unsigned long long le = lt | (1ull << lane);
+    // set bits of a wave mask below this lane (v_mbcnt)
+    auto below = [](unsigned long long m) -> int {
+        return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    };
 
     // kept cells: in range of the tile box and not empty, in cell order
     int K = 0, T = 0;
@@ -1255,7 +1257,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             const int incl = wave_incl_scan(cnt);
             const unsigned long long m = __ballot(cnt > 0);
             if (cnt > 0) {
-                const int k = K + (int)__popcll(m & lt);
+                const int k = K + below(m);
                 const int st = T + incl - cnt;
                 cstart[k] = st;
                 crec[k] = make_float4(lx, ly, lz, __int_as_float(b - st));
@@ -1312,7 +1314,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             if (s_hi >= t_next && s_hi < t_next + 64) sflag[s_hi - t_next] = 1;
             __syncthreads();
             const unsigned long long sm = __ballot(sflag[lane] != 0);
-            const int k = kbase + (int)__popcll(sm & le) - 1;
+            const int k = kbase + below(sm) + ((sm >> lane) & 1ull ? 1 : 0) - 1;
             kbase += (int)__popcll(sm);
             const int t = t_next + lane;
             bool keep = false;
@@ -1346,7 +1348,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             }
             const unsigned long long m = __ballot(keep);
             if (keep) {
-                const int p = n + (int)__popcll(m & lt);
+                const int p = n + below(m);
                 sA[REC * p] = r0;
                 sA[REC * p + 1] = r1;
                 if (ROW) sA[REC * p + 2] = r2;
