@@ -94,6 +94,7 @@ struct Workspace {
     Buf x;                      // atoms (temporary upload)
     Buf mask, blk_cnt, blk_off;
     Buf pt_cell, pt_slot, pt_frac, pt_gdotr;
+    Buf pt_p32, pt_sn32;        // FAST records in point order (FAST imaging)
     Buf cell_cnt, cell_start, cell_ids;
     Buf slot_id, slot_frac, slot_cs64, slot_p32, slot_sn32;
     Buf scan_tmp;
@@ -112,7 +113,7 @@ struct Workspace {
     size_t pinned_cap = 0;
     void release_all()
     {
-        Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr,
+        Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_p32, &pt_sn32,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
                       &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
@@ -298,6 +299,7 @@ struct BinParams {
     int Ex, Ey, Ez;
     int64_t n;                  // atoms
     int generic;                // 1: LinkCell3D add1 (no transform, no filter)
+    int fast;                   // FAST imaging: FP32 records instead of g.r and the FP64 slot arrays
     double cell_a[3];           // diagonal of the link-cell matrix (FAST-mode records)
     // replica pre-classification: c ~= M x + off[nn] approximates the
     // link-cell coordinates of replica nn (products folded, so a few
@@ -421,6 +423,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                                                            int* __restrict__ pt_slot,
                                                            double* __restrict__ pt_frac,
                                                            double* __restrict__ pt_gdotr,
+                                                           float4* __restrict__ pt_p32,
+                                                           float* __restrict__ pt_sn32,
                                                            int* __restrict__ cell_cnt)
 {
     __shared__ int run[MAX_REPL];
@@ -455,10 +459,22 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                 const int cz = P.generic ? cell_of(fl[2], P.Ez) : (int)fl[2];
                 int cell = cx + P.Ex * (cy + P.Ey * cz);
                 pt_cell[ii] = cell;
-                pt_frac[3 * (int64_t)ii + 0] = cyy[0] - fl[0];
-                pt_frac[3 * (int64_t)ii + 1] = cyy[1] - fl[1];
-                pt_frac[3 * (int64_t)ii + 2] = cyy[2] - fl[2];
-                pt_gdotr[ii] = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
+                const double f0 = cyy[0] - fl[0], f1 = cyy[1] - fl[1], f2 = cyy[2] - fl[2];
+                pt_frac[3 * (int64_t)ii + 0] = f0;
+                pt_frac[3 * (int64_t)ii + 1] = f1;
+                pt_frac[3 * (int64_t)ii + 2] = f2;
+                const double gd = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
+                if (P.fast) {
+                    // the FAST record: offset from the cell origin in A and
+                    // (cos, sin) of g.r, evaluated in FP64 and rounded once
+                    double sn, cs;
+                    sincos(gd, &sn, &cs);
+                    pt_p32[ii] = make_float4((float)(P.cell_a[0] * f0), (float)(P.cell_a[1] * f1),
+                                             (float)(P.cell_a[2] * f2), (float)cs);
+                    pt_sn32[ii] = (float)sn;
+                } else {
+                    pt_gdotr[ii] = gd;
+                }
                 // arrival slot: lanes of this wave landing in one cell share
                 // one atomic (the lowest such lane adds the group's size);
                 // the grouping is ALU-only, so each lane waits for one atomic
@@ -622,6 +638,20 @@ __global__ __launch_bounds__(256) void k_bin_gather(int64_t nr, const int* __res
     slot_sn32[s] = (float)sn;
 }
 
+// Pass 4b, FAST imaging: the point-ordered FAST records copied into slot
+// order (the FP64 offsets stay in point order for the guard re-test).
+__global__ __launch_bounds__(256) void k_bin_gather_fast(int64_t nr, const int* __restrict__ cell_ids,
+                                                         const float4* __restrict__ pt_p32,
+                                                         const float* __restrict__ pt_sn32,
+                                                         float4* __restrict__ slot_p32, float* __restrict__ slot_sn32)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nr) return;
+    const int ii = cell_ids[s];
+    slot_p32[s] = pt_p32[ii];
+    slot_sn32[s] = pt_sn32[ii];
+}
+
 struct Binned {
     int64_t nr = 0;
     int ncell = 0;
@@ -636,7 +666,11 @@ static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mp
     hipLaunchKernelGGL(k_bin_sort, dim3(nb), dim3(256), 0, s, ncell, w.cell_start.as<int>(), w.cell_ids.as<int>(),
                        w.blk_max.as<int>());
     hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(256), 0, s, w.blk_max.as<int>(), nb, mpc);
-    if (nr > 0)
+    if (nr > 0 && P.fast)
+        hipLaunchKernelGGL(k_bin_gather_fast, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
+                           w.cell_ids.as<int>(), w.pt_p32.as<float4>(), w.pt_sn32.as<float>(),
+                           w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
+    else if (nr > 0)
         hipLaunchKernelGGL(k_bin_gather, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
                            w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(), P.cell_a[0],
                            P.cell_a[1], P.cell_a[2], w.slot_id.as<int>(), w.slot_frac.as<double4>(),
@@ -679,18 +713,23 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     CHK(w.pt_cell.ensure(sizeof(int) * nrc));
     CHK(w.pt_slot.ensure(sizeof(int) * nrc));
     CHK(w.pt_frac.ensure(sizeof(double) * 3 * nrc));
-    CHK(w.pt_gdotr.ensure(sizeof(double) * nrc));
     CHK(w.cell_ids.ensure(sizeof(int) * nrc));
-    CHK(w.slot_id.ensure(sizeof(int) * nrc));
-    CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
-    CHK(w.slot_cs64.ensure(sizeof(double2) * nrc));
     CHK(w.slot_p32.ensure(sizeof(float4) * nrc));
     CHK(w.slot_sn32.ensure(sizeof(float) * nrc));
+    if (P.fast) {
+        CHK(w.pt_p32.ensure(sizeof(float4) * nrc));
+        CHK(w.pt_sn32.ensure(sizeof(float) * nrc));
+    } else {
+        CHK(w.pt_gdotr.ensure(sizeof(double) * nrc));
+        CHK(w.slot_id.ensure(sizeof(int) * nrc));
+        CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
+        CHK(w.slot_cs64.ensure(sizeof(double2) * nrc));
+    }
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
                            w.mask.as<uint32_t>(), w.blk_off.as<int>(), nblk, w.pt_cell.as<int>(),
                            w.pt_slot.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(),
-                           w.cell_cnt.as<int>());
+                           w.pt_p32.as<float4>(), w.pt_sn32.as<float>(), w.cell_cnt.as<int>());
         HIPCHK(hipGetLastError());
     }
     CHK(scan_exclusive(w.cell_cnt.as<int>(), w.cell_start.as<int>(), ncell, s));
@@ -1141,7 +1180,8 @@ struct FastParams {
 // tests the z offset of every pair when F.thin_z is set.
 template <bool DENSITY, bool DIAG, int ROW>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
-                                                   const double4* __restrict__ frac,
+                                                   const int* __restrict__ cell_ids,
+                                                   const double* __restrict__ pt_frac,
                                                    const float4* __restrict__ p32,
                                                    const float* __restrict__ sn32,
                                                    float4* __restrict__ field,
@@ -1281,11 +1321,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     auto exact_at = [&](int lx, int vx, int u, int slot) -> bool {
         const int kx = (u & 255) - 1 - lx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
         if (kx < -1 || kx > 1 || ky < -1 || ky > 1 || kz < -1 || kz > 1) return false;
-        const double4 f = frac[slot];
+        const double* f = pt_frac + 3 * (int64_t)cell_ids[slot];     // point-ordered FP64 offsets
         const double zix = P.ax.zi[0][vx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
-        const double d1 = P.a[0] * (f.x + (kx - zix));
-        const double d2 = P.a[1] * (f.y + (ky - ziy));
-        const double d3 = P.a[2] * (f.z + (kz - ziz));
+        const double d1 = P.a[0] * (f[0] + (kx - zix));
+        const double d2 = P.a[1] * (f[1] + (ky - ziy));
+        const double d3 = P.a[2] * (f[2] + (kz - ziz));
         return ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
     };
     auto kahan_add = [](float& s, float& c, float v) {
@@ -1872,13 +1912,13 @@ static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
 }
 
 template <bool DIAG>
-static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start, const double4* frac,
-                        const float4* p32, const float* sn32, float4* field, int* cnt, float* r2m,
-                        unsigned long long* gc, hipStream_t s)
+static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start, const int* cell_ids,
+                        const double* pt_frac, const float4* p32, const float* sn32, float4* field, int* cnt,
+                        float* r2m, unsigned long long* gc, hipStream_t s)
 {
     pick_tile(DIAG, dens, variant, F.thin_z != 0, [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, frac, p32, sn32, field, cnt,
-                           r2m, gc);
+        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, cell_ids, pt_frac, p32, sn32,
+                           field, cnt, r2m, gc);
     });
 }
 
@@ -1963,9 +2003,8 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     unsigned long long* gp = w.guard_part.as<unsigned long long>();
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, w.cell_start.as<int>(),
-                           w.slot_frac.as<double4>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>(),
-                           w.f32_field.as<float4>(), dcnt,
-                           (float*)dr2, gp);
+                           w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
+                           w.slot_sn32.as<float>(), w.f32_field.as<float4>(), dcnt, (float*)dr2, gp);
     };
     const bool dens = r->use_density != 0;
     auto run_variant = [&](int v) { pick_tile(want_diag, dens, v, F.thin_z != 0, launch); };
@@ -2082,6 +2121,7 @@ static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, d
     CHK(axis_tables(r, &plan, &tab));
     BinParams BP;
     CHK(bin_params(r, &plan, &BP));
+    BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
     hipStream_t s = g.stream;
     Timer T{g.ev, s};
     CHK(T.record(0));
@@ -2158,6 +2198,7 @@ static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, do
         for (int q = 0; q < nPrec; q++) pk.aa[q] = aa[app + (size_t)nApp * q];
         BinParams BP;
         CHK(bin_params(r, &pk, &BP));
+        BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
         Binned b;
         CHK(run_binning(d_x, n, BP, &b, s));
         CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s));
@@ -2297,7 +2338,8 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
         hipLaunchKernelGGL(k_bin_scatter, dim3(c.nblk), dim3(BIN_BLOCK), 0, s, c.d_x, c.BP,
                            B.mask.as<uint32_t>() + c.o_mask, B.blkoff.as<int>() + c.o_blk, c.nblk,
                            w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.pt_frac.as<double>(),
-                           w.pt_gdotr.as<double>(), w.cell_cnt.as<int>());
+                           w.pt_gdotr.as<double>(), w.pt_p32.as<float4>(), w.pt_sn32.as<float>(),
+                           w.cell_cnt.as<int>());
     CHK(scan_exclusive_tmp(w.cell_cnt.as<int>(), w.cell_start.as<int>(), c.ncell, scan_tmp, s));
     if (c.nr > 0)
         hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((c.nr + 255) / 256)), dim3(256), 0, s, c.nr,
@@ -2319,7 +2361,7 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
     }
     FastParams F;
     CHK(fast_params(p, c.tab, V, &F));
-    launch_tile<false>(F, c.variant, c.dens, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
+    launch_tile<false>(F, c.variant, c.dens, w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
                        gp, s);
     HIPCHK(hipGetLastError());
@@ -2362,6 +2404,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             CHK(plan_image(&recs[i], atoms[i]->n, precAngle[i], nPrec[i], &c.plan));
             CHK(axis_tables(&recs[i], &c.plan, &c.tab));
             CHK(bin_params(&recs[i], &c.plan, &c.BP));
+            c.BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
             c.d_x = atoms[i]->x;
             c.n = atoms[i]->n;
             c.BP.n = c.n;
@@ -2441,6 +2484,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         CHK(w.pt_slot.ensure(sizeof(int) * max_nr));
         CHK(w.pt_frac.ensure(sizeof(double) * 3 * max_nr));
         CHK(w.pt_gdotr.ensure(sizeof(double) * max_nr));
+        CHK(w.pt_p32.ensure(sizeof(float4) * max_nr));
+        CHK(w.pt_sn32.ensure(sizeof(float) * max_nr));
         CHK(w.cell_ids.ensure(sizeof(int) * max_nr));
         CHK(w.slot_id.ensure(sizeof(int) * max_nr));
         CHK(w.slot_frac.ensure(sizeof(double4) * max_nr));
@@ -2682,6 +2727,7 @@ int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t 
     CHK(axis_tables(rec, &plan, &tab));
     BinParams BP;
     CHK(bin_params(rec, &plan, &BP));
+    BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
     hipStream_t s = g.stream;
     AxisDev ax;
     CHK(upload_axis(tab, &ax, s));
