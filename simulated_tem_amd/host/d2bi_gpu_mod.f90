@@ -49,7 +49,7 @@ module d2bi_gpu_mod
     public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_plan_image
     public :: d2bi_setup_tilt_sg, d2bi_setup_tilt_ur, d2bi_setup_tilt_ng, d2bi_rotate_to_standard_frame
     public :: d2bi_gpu_atoms_upload, d2bi_gpu_atoms_free, d2bi_gpu_image_dev, d2bi_gpu_blur
-    public :: d2bi_gpu_image_aperture
+    public :: d2bi_gpu_image_aperture, d2bi_gpu_image_batch
     public :: d2bi_write_png16, d2bi_error_message
 
     interface
@@ -135,6 +135,17 @@ module d2bi_gpu_mod
             integer(c_int64_t), value       :: ld, nAtoms
             real(c_double), value           :: precAngle, apertureAngle
             integer(c_int32_t), value       :: nPrec, nAperture, mode
+            real(c_double), intent(out)     :: img(*)
+            type(d2bi_stats), intent(out)   :: stats
+        end function
+        integer(c_int) function d2bi_gpu_image_batch(recs, atoms, nRec, precAngle, nPrec, mode, img, stats) &
+                bind(c, name="d2bi_gpu_image_batch")
+            import :: c_int, c_int32_t, c_double, c_ptr, d2bi_record, d2bi_stats
+            type(d2bi_record), intent(in)   :: recs(*)
+            type(c_ptr), intent(in)         :: atoms(*)
+            integer(c_int32_t), value       :: nRec, mode
+            real(c_double), intent(in)      :: precAngle(*)
+            integer(c_int32_t), intent(in)  :: nPrec(*)
             real(c_double), intent(out)     :: img(*)
             type(d2bi_stats), intent(out)   :: stats
         end function

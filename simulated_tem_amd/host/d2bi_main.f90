@@ -9,7 +9,9 @@
 !   -precAngle -nTomoAngle -tomoAngle -density -M -xpad -ypad -zpad -d
 !   -addOffset -U -R -xifile -xi0 -xig -ng -png -png_min -png_max -png_blur,
 !   plus -mode fp64|fast and -device n.  -alterg/-dp, -surf and hcp are
-!   not part of this build and stop with a message.
+!   not part of this build and stop with a message.  A tomography series
+!   (-nTomoAngle > 1) is imaged as one batch (d2bi_gpu_image_batch: one
+!   captured graph for every tilt); -nobatch images the tilts one by one.
 program d2bi_host
     use iso_fortran_env, only: real64, int64, output_unit
     use iso_c_binding
@@ -32,7 +34,7 @@ program d2bi_host
     real(real64) :: d0(3), addOffset(3), U(3, 3), R(3, 3), dat(12)
     integer      :: M(2) = [4, 4], nPrec = 1, nTomo = 1, device = 0, mode
     logical      :: useDensity = .false., xpad = .false., ypad = .false., zpad = .false.
-    logical      :: opPng = .true., haveU = .false., haveR = .false.
+    logical      :: opPng = .true., haveU = .false., haveR = .false., useBatch = .true.
     integer      :: nDiff = 1, n, ii, nn, kk, rc, Mx, My, Mz, latcode
     real(real64), allocatable :: x(:, :), img(:, :), blurred(:, :)
     real(c_float), allocatable :: pngbuf(:)
@@ -43,6 +45,12 @@ program d2bi_host
     type(d2bi_plan)   :: plan
     type(d2bi_stats)  :: st
     type(c_ptr)       :: atoms
+    type(d2bi_record), allocatable :: recs(:)
+    type(c_ptr), allocatable       :: hs(:)
+    real(real64), allocatable      :: pas(:), imgs(:)
+    integer(c_int32_t), allocatable :: nps(:)
+    integer(int64), allocatable    :: offs(:)
+    real(real64) :: thetas(1000)
     logical      :: ok, dark
     character(len=1024) :: outfile, fname
     character(len=64)   :: s1, s2, s3
@@ -53,7 +61,7 @@ program d2bi_host
         write(*, '(a)') "usage: d2bi -f file -a0 a [-lattice bcc] -g h,k,l[,...] -k u,v,w [-sg s | -ng n] [-o prefix]"
         write(*, '(a)') "       [-dark|-nodark] [-V keV] [-M xy[,z]] [-d x,y,z] [-U 9 reals] [-R 9 reals]"
         write(*, '(a)') "       [-xifile f | -xi0 a -xig b] [-density] [-nPrecAngle n -precAngle rad]"
-        write(*, '(a)') "       [-nTomoAngle n -tomoAngle deg] [-xpad|-ypad|-zpad] [-png_blur A] [-mode fp64|fast]"
+        write(*, '(a)') "       [-nTomoAngle n -tomoAngle deg [-nobatch]] [-xpad|-ypad|-zpad] [-png_blur A] [-mode fp64|fast]"
         stop 1
     end if
     call cla_string("f", filename)
@@ -145,6 +153,8 @@ program d2bi_host
     call cla_ints("nPrecAngle", sN(1:1), n); if (n > 0) nPrec = sN(1)
     call cla_reals("precAngle", dat, n); if (n > 0) precAngle = dat(1)
     call cla_ints("nTomoAngle", sN(1:1), n); if (n > 0) nTomo = sN(1)
+    if (cla_has("nobatch")) useBatch = .false.
+    if (nTomo > size(thetas)) error stop "d2bi error - at most 1000 tomography steps"
     call cla_reals("tomoAngle", dat, n); if (n > 0) tomoAngle = dat(1)
     bl(1) = .false.; call cla_logical("density", bl, n); if (n > 0) useDensity = bl(1)
     call cla_ints("M", M, n)
@@ -319,6 +329,10 @@ program d2bi_host
             if (nPrec > 1) outfile = trim(outfile)//"_p"
         end if
 
+        if (nTomo > 1 .and. useBatch) then
+            call tomography_batch()
+            cycle
+        end if
         do kk = 0, nTomo - 1
             if (nTomo > 1) then
                 theta = tomoAngle - 2 * tomoAngle * kk / (nTomo - 1)
@@ -370,6 +384,70 @@ program d2bi_host
     write(*, '(a)') ""
 
 contains
+
+    !   The tomography series of one diffraction condition as one batch: the
+    !   records of every tilt R = Rx(theta) R0 (d2bi.f90:1352-1372), imaged by
+    !   d2bi_gpu_image_batch, then written step by step as the sequential loop
+    !   writes them.
+    subroutine tomography_batch()
+        integer(int64) :: tot
+        real(real64)   :: ms
+        allocate(recs(nTomo), hs(nTomo), pas(nTomo), nps(nTomo), offs(nTomo + 1))
+        tot = 0
+        do kk = 0, nTomo - 1
+            theta = tomoAngle - 2 * tomoAngle * kk / (nTomo - 1)
+            thetas(kk + 1) = theta
+            write(*, '(a,i4,a,i4,a,f10.4,a)') " d2bi info - tomography step ", kk + 1, "/", nTomo, &
+                " theta = ", theta, " deg"
+            theta = theta * 3.141592654d0 / 180.0d0
+            Rt = reshape([1.0d0, 0.0d0, 0.0d0, 0.0d0, cos(theta), sin(theta), 0.0d0, -sin(theta), &
+                          cos(theta)], [3, 3])
+            Rt = matmul(Rt, R0)
+            recs(kk + 1) = rec
+            rc = d2bi_setup_tilt_ur(recs(kk + 1), reshape(U0, [9]), reshape(Rt, [9]), int(M(1), c_int32_t), &
+                                    int(M(2), c_int32_t), d)
+            if (rc /= 0) call fail("tomography setup")
+            rc = d2bi_plan_image(recs(kk + 1), int(size(x, 2), c_int64_t), precAngle, int(nPrec, c_int32_t), plan)
+            if (rc /= 0) call fail("plan")
+            offs(kk + 1) = tot
+            tot = tot + int(plan%Mx, int64) * plan%My
+            hs(kk + 1) = atoms
+            pas(kk + 1) = precAngle
+            nps(kk + 1) = int(nPrec, c_int32_t)
+        end do
+        offs(nTomo + 1) = tot
+        allocate(imgs(tot))
+        rc = d2bi_gpu_image_batch(recs, hs, int(nTomo, c_int32_t), pas, nps, int(mode, c_int32_t), imgs, st)
+        if (rc /= 0) call fail("computeD2BI_img (batch)")
+        t_d2bi = t_d2bi + st%ms_total / 1000
+        ms = st%ms_total
+        write(*, '(a,i6,a,f10.3,a)') " d2bi info - tomography batch of ", nTomo, " tilts, GPU time ", ms, " ms"
+        do kk = 0, nTomo - 1
+            rec = recs(kk + 1)                 ! the step's record, as the sequential loop leaves it
+            rc = d2bi_plan_image(recs(kk + 1), int(size(x, 2), c_int64_t), precAngle, int(nPrec, c_int32_t), plan)
+            Mx = plan%Mx; My = plan%My; Mz = plan%Mz
+            if (allocated(img)) deallocate(img)
+            allocate(img(0:Mx - 1, 0:My - 1))
+            img = reshape(imgs(offs(kk + 1) + 1:offs(kk + 2)), [Mx, My])
+            if (.not. dark) img = 1 - img
+            write(*, '(a,i4,a,f10.4,a,3i6)') " d2bi info - tomography step ", kk + 1, " theta = ", thetas(kk + 1), &
+                " deg  voxels ", Mx, My, Mz
+            if (.not. dark) then
+                write(*, '(a,3f12.6,a)') "d2bi info - minmaxavg d2bimg     ", minval(img), maxval(img), &
+                    sum(img) / (size(img, 1) * size(img, 2)), " (transmitted beam) "
+            else
+                write(*, '(a,3f12.6,a)') "d2bi info - minmaxavg d2bimg     ", minval(img), maxval(img), &
+                    sum(img) / (size(img, 1) * size(img, 2)), " (diffracted beam) "
+            end if
+            if (len_trim(outfile0) > 0) then
+                write(s1, '(i4.4)') kk
+                fname = trim(outfile)//"."//trim(s1)
+                call write_dat(trim(fname), img, plan)
+                if (opPng) call write_png(trim(fname), img, plan%aperpx)
+            end if
+        end do
+        deallocate(recs, hs, pas, nps, offs, imgs)
+    end subroutine tomography_batch
 
     subroutine fail(what)
         character(len=*), intent(in) :: what

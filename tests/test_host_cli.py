@@ -144,3 +144,31 @@ def test_golden_run_through_host(tmp_path, golden_dir):
         png = _read_png16(tmp_path / (dats[0][:-4] + ".png"))
         assert png.shape == img.shape
         assert png.min() == 0 and png.max() >= 65534     # normalised output spans the range
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_tomography_series_batched_equals_sequential(tmp_path, golden_dir):
+    """-nTomoAngle 3 on the W loop: the series imaged as one batch (one
+    captured graph for the three tilts) writes the same .dat files, byte for
+    byte, as the tilts imaged one at a time (-nobatch), in both modes; the
+    tilts differ from each other."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    write_lammps(tmp_path / "w.lammps", w["x"], w["box"])
+    write_xi(tmp_path / "xi.dat", w["xi"][0], w["xi"][1], (2, 0, 0))
+    base = ["-f", "w.lammps", "-a0", "3.1652", "-lattice", "bcc", "-k", "0,0,1", "-g", "2,0,0",
+            "-xifile", "xi.dat", "-sg", "0.02", "-nTomoAngle", "3", "-tomoAngle", "2", "-nopng"]
+    for mode in ("fp64", "fast"):
+        outs = {}
+        for tag, extra in (("batch", []), ("seq", ["-nobatch"])):
+            r = run(base + ["-o", tag + "_" + mode, "-mode", mode] + extra, tmp_path)
+            assert r.returncode == 0, r.stdout + r.stderr
+            dats = sorted(f for f in os.listdir(tmp_path) if f.startswith(tag + "_" + mode) and f.endswith(".dat"))
+            assert len(dats) == 3, dats
+            outs[tag] = [open(tmp_path / f).read() for f in dats]
+            mm = re.findall(r"minmaxavg d2bimg\s+(\S+)\s+(\S+)\s+(\S+)", r.stdout)
+            assert len(mm) == 3
+        for a, b in zip(outs["batch"], outs["seq"]):
+            assert a.splitlines()[1:] == b.splitlines()[1:]       # first line names the file
+        assert outs["batch"][0].splitlines()[-5:] != outs["batch"][2].splitlines()[-5:]
+        print("[tomography] %s: batch == sequential for 3 tilts" % mode, flush=True)
