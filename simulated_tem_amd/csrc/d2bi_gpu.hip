@@ -833,7 +833,7 @@ struct QL {
 struct AxisTables {
     std::vector<int> cell[3];
     std::vector<double> zi[3];
-    std::vector<QL> ql[3];
+    std::vector<QL> ql[4];         // x, y, z for 4-voxel tiles; z for 8-voxel tiles
     std::vector<int> start[3];     // first voxel of each link cell (size E+1)
     int lo[3], hi[3];              // active link-cell range
 };
@@ -870,6 +870,14 @@ static int axis_tables(const d2bi_record* r, const d2bi_plan* p, AxisTables* t)
             const int lc = t->cell[a][i] - t->cell[a][(i / TILE) * TILE];
             t->ql[a][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
         }
+        if (a == 2) {
+            // z again, relative to the cell of 8-voxel tile starts (4 x 4 x 8 tiles)
+            t->ql[3].resize(M[a]);
+            for (int i = 0; i < M[a]; i++) {
+                const int lc = t->cell[a][i] - t->cell[a][(i / 8) * 8];
+                t->ql[3][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
+            }
+        }
         for (int i = M[a] - 1; i >= 0; i--) t->start[a][t->cell[a][i]] = i;
         // cells with no voxels take the start of the next occupied cell
         for (int c = E[a] - 1; c >= 0; c--)
@@ -885,7 +893,7 @@ struct AxisDev {
     const int* cell[3];
     const double* zi[3];
     const int* start[3];
-    const QL* ql[3];
+    const QL* ql[4];
 };
 
 static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
@@ -893,11 +901,12 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
     Workspace& w = g.ws;
     size_t nc = t.cell[0].size() + t.cell[1].size() + t.cell[2].size();
     size_t ns = t.start[0].size() + t.start[1].size() + t.start[2].size();
+    size_t nq = nc + t.ql[3].size();
     CHK(w.ax_cell.ensure(sizeof(int) * nc));
     CHK(w.ax_zi.ensure(sizeof(double) * nc));
     CHK(w.ax_start.ensure(sizeof(int) * ns));
-    CHK(w.ax_ql.ensure(sizeof(QL) * nc));
-    size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns + sizeof(QL) * nc;
+    CHK(w.ax_ql.ensure(sizeof(QL) * nq));
+    size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns + sizeof(QL) * nq;
     void* hp;
     CHK(pinned(bytes, &hp));
     char* h = (char*)hp;
@@ -912,6 +921,10 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
         std::memcpy(hs + os, t.start[a].data(), sizeof(int) * t.start[a].size());
         std::memcpy(hq + oc, t.ql[a].data(), sizeof(QL) * t.ql[a].size());
         d->ql[a] = w.ax_ql.as<QL>() + oc;
+        if (a == 2) {
+            std::memcpy(hq + nc, t.ql[3].data(), sizeof(QL) * t.ql[3].size());
+            d->ql[3] = w.ax_ql.as<QL>() + nc;
+        }
         d->cell[a] = w.ax_cell.as<int>() + oc;
         d->zi[a] = w.ax_zi.as<double>() + oc;
         d->start[a] = w.ax_start.as<int>() + os;
@@ -921,7 +934,7 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
     HIPCHK(hipMemcpyAsync(w.ax_cell.p, hc, sizeof(int) * nc, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.ax_zi.p, hz, sizeof(double) * nc, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.ax_start.p, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(w.ax_ql.p, hq, sizeof(QL) * nc, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.ax_ql.p, hq, sizeof(QL) * nq, hipMemcpyHostToDevice, s));
     // the pinned buffer is reused by later calls: they wait for this event
     HIPCHK(hipEventRecord(g.ev[9], s));
     g.pinned_busy = true;
@@ -1121,6 +1134,7 @@ struct FastParams {
     float inv_k2;               // 1 / k2
     float cellf[3];             // link-cell sides in FP32
     int thin_z;                 // link cells thinner than 3 sigma along z
+    int tz;                     // tile depth in voxels (4, or 8 for the 4 x 4 x 8 row kernel)
 };
 
 // One wave per 4x4x4 voxel tile.  The tile's candidates are the points of
@@ -1178,7 +1192,7 @@ struct FastParams {
 // see it.  ROW = 2 applies that rule: a candidate whose cell is not within one
 // cell of the row's along z gets weight 0 for the row.  The diagnostic kernel
 // tests the z offset of every pair when F.thin_z is set.
-template <bool DENSITY, bool DIAG, int ROW>
+template <bool DENSITY, bool DIAG, int ROW, int TZ>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
                                                    const int* __restrict__ cell_ids,
                                                    const double* __restrict__ pt_frac,
@@ -1190,6 +1204,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                                                    unsigned long long* __restrict__ guard_count)
 {
     static_assert(!(DIAG && ROW), "the diagnostic kernel tests every pair itself");
+    static_assert(TZ == 4 || (ROW && TZ == 8), "tiles of 4 x 4 x 4, or 4 x 4 x 8 for the row kernel");
+    // ROW: candidate parts per row (4 for 16 rows, 2 for 32 rows), voxels a
+    // lane finishes (4 / NQ)
+    constexpr int NQ = ROW ? (TZ == 8 ? 2 : 4) : 1;
+    constexpr int NV = ROW ? 4 / NQ : 1;
     // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
     // give each XCD a contiguous range of tiles so neighbouring tiles, which
     // read the same link cells, share one L2.
@@ -1199,17 +1218,20 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const VoxParams& P = F.v;
     const int tx = L % F.ntx, ty = (L / F.ntx) % F.nty, tz = L / (F.ntx * F.nty);
     const int lane = threadIdx.x;
-    const int x0 = tx * TILE, y0 = ty * TILE, z0 = tz * TILE;
-    const int x1 = min(P.Mx, x0 + TILE) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TILE) - 1;
-    const int ix = x0 + (lane & 3), iy = y0 + ((lane >> 2) & 3), iz = z0 + (lane >> 4);
+    const int x0 = tx * TILE, y0 = ty * TILE, z0 = tz * TZ;
+    const int x1 = min(P.Mx, x0 + TILE) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TZ) - 1;
+    // the lane's row of voxels along x (y fastest) and, lane per voxel, its x
+    const int rrow = ROW ? lane / NQ : (lane >> 2);
+    const int ix = x0 + (lane & 3), iy = y0 + (rrow & 3), iz = z0 + (rrow >> 2);
     const bool row_act = iy <= y1 && iz <= z1;
     const bool act = ix <= x1 && row_act;
     const int jx = min(ix, x1), jy = min(iy, y1), jz = min(iz, z1);
+    const QL* const qlz_tab = TZ == 8 ? P.ax.ql[3] : P.ax.ql[2];   // z relative to the tile's first cell
 
     // origin cell of the tile; the voxel's FP32 position relative to it and
     // its cell offset (host tables)
     const int c0x = P.ax.cell[0][x0], c0y = P.ax.cell[1][y0], c0z = P.ax.cell[2][z0];
-    const QL qlx = P.ax.ql[0][jx], qly = P.ax.ql[1][jy], qlz = P.ax.ql[2][jz];
+    const QL qlx = P.ax.ql[0][jx], qly = P.ax.ql[1][jy], qlz = qlz_tab[jz];
     const int lcx = qlx.lc, lcy = qly.lc, lcz = qlz.lc;
     const float qx = qlx.q, qy = qly.q, qz = qlz.q;
     // the tile's voxel box: positions do not decrease with the voxel index
@@ -1217,7 +1239,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     // first and last voxels
     const QL ex0 = P.ax.ql[0][x0], ex1 = P.ax.ql[0][x1];
     const QL ey0 = P.ax.ql[1][y0], ey1 = P.ax.ql[1][y1];
-    const QL ez0 = P.ax.ql[2][z0], ez1 = P.ax.ql[2][z1];
+    const QL ez0 = qlz_tab[z0], ez1 = qlz_tab[z1];
     const float bx0 = ex0.q, bx1 = ex1.q, by0 = ey0.q, by1 = ey1.q, bz0 = ez0.q, bz1 = ez1.q;
     // tile-centred frame
     const float ccx = 0.5f * (bx0 + bx1), ccy = 0.5f * (by0 + by1), ccz = 0.5f * (bz0 + bz1);
@@ -1226,11 +1248,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const float k2 = F.k2;
     const float kq2 = k2 * fmaf(qcx, qcx, fmaf(qcy, qcy, qcz * qcz));
     // ROW: the tile's four x positions (clamped at the grid's edge), the
-    // lane's candidate quarter and the y-z part of its exponent
+    // lane's candidate part and the y-z part of its exponent
     float qxt[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) qxt[i] = P.ax.ql[0][min(x0 + i, x1)].q - ccx;
-    const int squart = lane & 3;
+    const int squart = lane & (NQ - 1);
     const float kyz = fmaf(k2, fmaf(qcy, qcy, qcz * qcz), -F.e_rc);
 
     // unwrapped cell range (cells are monotonic, so ux0 + dx lies in [-1, E])
@@ -1472,16 +1494,16 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 kahan_add(sim, cim, pi);
             }
         } else {
-            // ROW: steps of four candidates (one per quarter), blocks of 8
-            // steps; the guard band in u is the exponent's band (|u| ~ ln2 |t|
-            // near the threshold, so it is 1.44 times as wide)
-            const int nst = (n + 3) >> 2;
+            // ROW: steps of NQ candidates (one per part), blocks of 8 steps;
+            // the guard band in u is the exponent's band (|u| ~ ln2 |t| near
+            // the threshold, so it is 1.44 times as wide)
+            const int nst = (n + NQ - 1) / NQ;
             for (int kb = 0; kb < nst; kb += 8) {
                 const int ke = min(kb + 8, nst);
                 float dmin = 3.4028234663852886e38f;
 #pragma unroll 2
                 for (int kk = kb; kk < ke; kk++) {
-                    const int j = 4 * kk + squart;
+                    const int j = NQ * kk + squart;
                     const float4 X = sA[REC * j];
                     const float4 B = sA[REC * j + 1];
                     float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
@@ -1507,7 +1529,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 }
                 if (__ballot(dmin <= F.e_band)) {
                     for (int kk = kb; kk < ke; kk++) {
-                        const int j = 4 * kk + squart;
+                        const int j = NQ * kk + squart;
                         const float4 X = sA[REC * j];
                         const float4 B = sA[REC * j + 1];
                         const float4 C = sA[REC * j + 2];
@@ -1537,9 +1559,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
         __syncthreads();
     }
 
+    // the lane's voxels: ROW, voxels s NV .. s NV + NV - 1 of its row, from
+    // the parts' sums added in part order through LDS; otherwise its own
+    float vre[NV], vim[NV], vem[NV];
     if (ROW) {
-        // quarters' sums of each voxel, added in quarter order; lane (row, s)
-        // takes voxel s of its row
         float2* red = reinterpret_cast<float2*>(sA);                  // [64 lanes][4 voxels]
         float* redm = reinterpret_cast<float*>(sA) + 2 * 256;         // [64 lanes][4 voxels]
 #pragma unroll
@@ -1548,27 +1571,39 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             if (DENSITY) redm[4 * lane + i] = umax[i];
         }
         __syncthreads();
-        const int rb = lane & ~3;
-        float a = 0.0f, b = 0.0f, um = -1.0f;
+        const int rb = lane & ~(NQ - 1);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float2 v = red[4 * (rb + q) + squart];
-            a += v.x;
-            b += v.y;
-            if (DENSITY) um = fmaxf(um, redm[4 * (rb + q) + squart]);
+        for (int t = 0; t < NV; t++) {
+            const int i = squart * NV + t;
+            float a = 0.0f, b = 0.0f, um = -1.0f;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const float2 v = red[4 * (rb + q) + i];
+                a += v.x;
+                b += v.y;
+                if (DENSITY) um = fmaxf(um, redm[4 * (rb + q) + i]);
+            }
+            vre[t] = a;
+            vim[t] = b;
+            // largest exponent in the t frame: 2^t = u + 1 (u = -1: no candidate)
+            vem[t] = (DENSITY && um > -1.0f) ? __builtin_amdgcn_logf(um + 1.0f) : -3.4028234663852886e38f;
         }
-        sre = a;
-        sim = b;
-        // largest exponent in the t frame: 2^t = u + 1 (u = -1: no candidate)
-        if (DENSITY && um > -1.0f) emax = __builtin_amdgcn_logf(um + 1.0f);
+    } else {
+        vre[0] = sre;
+        vim[0] = sim;
+        vem[0] = emax;
     }
 
-    if (act) {
-        const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+        const int vx = ROW ? x0 + squart * NV + t : ix;
+        if (!(ROW ? vx <= x1 && row_act : act)) continue;
+        const int64_t o = (int64_t)vx + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
+        const float sre_ = vre[t], sim_ = vim[t], emax_ = vem[t];
         // x = S / max(|S|, 1e-16).  Without DIAG the sums carry the factor
         // 2^-e_rc of the t frame (weights 2^t = 2^(e - e_rc)); it cancels in
         // S/|S| and moves the threshold.
-        const float m2 = fmaf(sre, sre, sim * sim);
+        const float m2 = fmaf(sre_, sre_, sim_ * sim_);
         const float ww = m2 > (DIAG ? 1.0e-32f : F.m2_min) ? __builtin_amdgcn_rsqf(m2)
                                                           : (DIAG ? 1.0e16f : 1.0e16f * F.w_scale);
         // nearest squared range from the largest exponent (huge(1.0) if none).
@@ -1576,21 +1611,21 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
         // or not: it differs from the members' only when no candidate is
         // nearer than about 3 sigma, where the density is 0 either way.
         float r2m;
-        if (DIAG) r2m = emax > -3.4028234663852886e38f ? emax / k2 : 3.4028234663852886e38f;
-        else r2m = emax > -3.4028234663852886e38f ? (emax + F.e_rc) * F.inv_k2 : 3.4028234663852886e38f;
+        if (DIAG) r2m = emax_ > -3.4028234663852886e38f ? emax_ / k2 : 3.4028234663852886e38f;
+        else r2m = emax_ > -3.4028234663852886e38f ? (emax_ + F.e_rc) * F.inv_k2 : 3.4028234663852886e38f;
         float rh = 1.0f;
         if (DENSITY && P.use_density) {
             // convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994) in FP32
-            const float s = (float)P.sigma;
-            if (r2m < s * s) rh = 1.0f;
-            else if (r2m > 4 * s * s) rh = 0.0f;
+            const float sg = (float)P.sigma;
+            if (r2m < sg * sg) rh = 1.0f;
+            else if (r2m > 4 * sg * sg) rh = 0.0f;
             else {
-                const float xx = sqrtf(r2m) / s - 1.0f;
+                const float xx = sqrtf(r2m) / sg - 1.0f;
                 rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
             }
         }
-        if (DENSITY) field[o] = make_float4(sre * ww, sim * ww, rh, 0.0f);
-        else reinterpret_cast<float2*>(field)[o] = make_float2(sre * ww, sim * ww);
+        if (DENSITY) field[o] = make_float4(sre_ * ww, sim_ * ww, rh, 0.0f);
+        else reinterpret_cast<float2*>(field)[o] = make_float2(sre_ * ww, sim_ * ww);
         if (DIAG) {
             out_cnt[o] = cnt;
             out_r2m[o] = r2m;
@@ -1852,14 +1887,19 @@ static void fill_vox_params(const d2bi_record* r, const d2bi_plan* p, const Axis
     V->ax = ax;
 }
 
-// Launch parameters of the FAST voxel kernel for a plan.
-static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo)
+// Tile depth of a FAST kernel variant: 0 (a lane per voxel) and 1 (rows of
+// 4 x 4 x 4 tiles) use 4 voxels, 2 (rows of 4 x 4 x 8 tiles) 8.
+static int variant_tz(int variant) { return variant == 2 ? 8 : 4; }
+
+// Launch parameters of the FAST voxel kernel for a plan, tiles of 4 x 4 x tz.
+static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo, int tz)
 {
     FastParams& F = *Fo;
     F.v = V;
+    F.tz = tz;
     F.ntx = (p->Mx + TILE - 1) / TILE;
     F.nty = (p->My + TILE - 1) / TILE;
-    F.ntz = (p->Mz + TILE - 1) / TILE;
+    F.ntz = (p->Mz + tz - 1) / tz;
     F.ntiles = F.ntx * F.nty * F.ntz;
     F.per_xcd = (F.ntiles + 7) / 8;
     // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
@@ -1884,30 +1924,53 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     int worst = 1;
     for (int a = 0; a < 3; a++) {
         const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
+        const int ts = a == 2 ? tz : TILE;
         int mx = 0;
-        for (int t0 = 0; t0 < M; t0 += TILE)
-            mx = std::max(mx, tab.cell[a][std::min(M, t0 + TILE) - 1] - tab.cell[a][t0] + 3);
+        for (int t0 = 0; t0 < M; t0 += ts)
+            mx = std::max(mx, tab.cell[a][std::min(M, t0 + ts) - 1] - tab.cell[a][t0] + 3);
         worst *= mx;
     }
     if (worst > TILE_MAXC) return seterr(D2BI_ERR_CAPACITY, "voxel tile spans %d link cells (max %d)", worst, TILE_MAXC);
     return D2BI_OK;
 }
 
+// Launch parameters of a variant.  A 4 x 4 x 8 tile that spans more link
+// cells than the per-tile table holds falls back to the 4 x 4 x 4 row kernel
+// (*variant becomes 1).
+static int variant_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, int* variant,
+                          FastParams* F)
+{
+    if (*variant == 2) {
+        if (fast_params(p, tab, V, F, 8) == D2BI_OK) return D2BI_OK;
+        *variant = 1;
+    }
+    return fast_params(p, tab, V, F, 4);
+}
+
 // Kernel instance for (diagnostics, density, variant); link cells thinner
-// than 3 sigma along z take the row kernel with the z-offset test.
+// than 3 sigma along z take a row kernel (variant 1 or 2) with the z-offset
+// test.
+template <bool DENS, int ROW, class Go>
+static void pick_row(int tz, Go go)
+{
+    if (tz == 8) go(k_voxel_tile<DENS, false, ROW, 8>);
+    else go(k_voxel_tile<DENS, false, ROW, 4>);
+}
+
 template <class Go>
 static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
 {
-    if (diag) { go(k_voxel_tile<true, true, 0>); return; }
-    const int row = thin_z ? 2 : (variant ? 1 : 0);
-    if (dens) {
-        if (row == 2) go(k_voxel_tile<true, false, 2>);
-        else if (row == 1) go(k_voxel_tile<true, false, 1>);
-        else go(k_voxel_tile<true, false, 0>);
+    if (diag) { go(k_voxel_tile<true, true, 0, 4>); return; }
+    const int tz = variant_tz(variant);
+    if (!thin_z && variant == 0) {
+        if (dens) go(k_voxel_tile<true, false, 0, 4>);
+        else go(k_voxel_tile<false, false, 0, 4>);
+    } else if (thin_z) {
+        if (dens) pick_row<true, 2>(tz, go);
+        else pick_row<false, 2>(tz, go);
     } else {
-        if (row == 2) go(k_voxel_tile<false, false, 2>);
-        else if (row == 1) go(k_voxel_tile<false, false, 1>);
-        else go(k_voxel_tile<false, false, 0>);
+        if (dens) pick_row<true, 1>(tz, go);
+        else pick_row<false, 1>(tz, go);
     }
 }
 
@@ -1923,14 +1986,16 @@ static void launch_tile(const FastParams& F, int variant, bool dens, const int* 
 }
 
 // Choice of the FAST voxel kernel variant (0: a lane per voxel, 1: a lane
-// per row of four voxels and candidate quarter) per voxel-grid shape: timed once on the first
-// condition of that shape, then cached.  D2BI_FAST_VARIANT=0|1 forces one.
+// per row of four voxels and quarter of the candidates of a 4 x 4 x 4 tile,
+// 2: a lane per row and half of the candidates of a 4 x 4 x 8 tile) per
+// voxel-grid shape: timed once on the first condition of that shape, then
+// cached.  D2BI_FAST_VARIANT=0|1|2 forces one.
 static std::map<std::array<int, 7>, int> g_variant_cache;
 
 static int forced_variant()
 {
     const char* env = std::getenv("D2BI_FAST_VARIANT");
-    if (env && (env[0] == '0' || env[0] == '1')) return env[0] - '0';
+    if (env && env[0] >= '0' && env[0] <= '2' && env[1] == 0) return env[0] - '0';
     return -1;
 }
 
@@ -1945,7 +2010,7 @@ static int known_variant(const d2bi_plan* p, bool dens)
 }
 
 template <class TimeFn>
-static int fast_variant(const d2bi_plan* p, bool dens, bool diag, TimeFn time_variant)
+static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, TimeFn time_variant)
 {
     auto& cache = g_variant_cache;
     const int f = forced_variant();
@@ -1954,12 +2019,18 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, TimeFn time_va
     const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    // each variant twice, the second (warm-cache) timing counts
-    float t0 = time_variant(0), t1 = time_variant(1);
-    t0 = time_variant(0);
-    t1 = time_variant(1);
-    if (t0 < 0 || t1 < 0) return seterr(D2BI_ERR_HIP, "variant timing failed");
-    const int best = t1 < t0 ? 1 : 0;
+    // each variant twice, the second (warm-cache) timing counts; thin z
+    // cells have only the row kernels
+    const int v0 = thin_z ? 1 : 0;
+    float t[3] = {0.0f, 0.0f, 0.0f};
+    for (int rep = 0; rep < 2; rep++)
+        for (int v = v0; v < 3; v++) {
+            t[v] = time_variant(v);
+            if (t[v] < 0) return seterr(D2BI_ERR_HIP, "variant timing failed");
+        }
+    int best = v0;
+    for (int v = v0 + 1; v < 3; v++)
+        if (t[v] < t[best]) best = v;
     cache[key] = best;
     return best;
 }
@@ -1995,23 +2066,30 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     }
     CHK(w.f32_field.ensure(sizeof(float4) * nvox));
     FastParams F;
-    CHK(fast_params(p, tab, V, &F));
+    CHK(fast_params(p, tab, V, &F, 4));
     CHK(w.counters.ensure(64));
     unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
     const size_t gp_bytes = sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE;
     CHK(w.guard_part.ensure(gp_bytes));
     unsigned long long* gp = w.guard_part.as<unsigned long long>();
-    auto launch = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, w.cell_start.as<int>(),
-                           w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
-                           w.slot_sn32.as<float>(), w.f32_field.as<float4>(), dcnt, (float*)dr2, gp);
-    };
     const bool dens = r->use_density != 0;
-    auto run_variant = [&](int v) { pick_tile(want_diag, dens, v, F.thin_z != 0, launch); };
-    // thin z cells have one kernel (no tuning)
-    const int variant = F.thin_z ? 1 : fast_variant(p, dens, want_diag, [&](int v) -> float {
+    // launch parameters follow the variant's tile depth
+    auto run_variant = [&](int v) -> int {
+        FastParams Fv;
+        CHK(variant_params(p, tab, V, &v, &Fv));
+        pick_tile(want_diag, dens, v, Fv.thin_z != 0, [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(8 * Fv.per_xcd), dim3(64), 0, s, Fv, w.cell_start.as<int>(),
+                               w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
+                               w.slot_sn32.as<float>(), w.f32_field.as<float4>(), dcnt, (float*)dr2, gp);
+        });
+        return D2BI_OK;
+    };
+    const int variant = fast_variant(p, dens, want_diag, F.thin_z != 0, [&](int v) -> float {
+        // a variant whose tiles span too many link cells is not timed
+        FastParams Fv;
+        if (fast_params(p, tab, V, &Fv, variant_tz(v)) < 0) return 3.4e38f;
         HIPCHK(hipEventRecord(g.ev[7], s));
-        run_variant(v);
+        CHK(run_variant(v));
         HIPCHK(hipEventRecord(g.ev[8], s));
         HIPCHK(hipEventSynchronize(g.ev[8]));
         float ms = 0.0f;
@@ -2021,7 +2099,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     if (variant < 0) return variant;
     HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches counted too
     if (T) CHK(T->record(2));
-    run_variant(variant);
+    CHK(run_variant(variant));
     HIPCHK(hipGetLastError());
     if (T) CHK(T->record(3));
     hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
@@ -2275,7 +2353,7 @@ struct Cond {
     int nblk = 0, ncell = 0, variant = 0;
     bool dens = false;
     int64_t nr = 0, nbc = 0, nvox = 0, plane = 0;
-    int64_t o_mask = 0, o_blk = 0, o_ax = 0, o_axs = 0, o_img = 0, o_aa = 0;
+    int64_t o_mask = 0, o_blk = 0, o_ax = 0, o_axs = 0, o_axq = 0, o_img = 0, o_aa = 0;
     AxisDev ax;
 };
 
@@ -2360,8 +2438,9 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
         return launch_integrate_fp64(p, B.aa.as<double>() + c.o_aa, d_img, s);
     }
     FastParams F;
-    CHK(fast_params(p, c.tab, V, &F));
-    launch_tile<false>(F, c.variant, c.dens, w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
+    int variant = c.variant;
+    CHK(variant_params(p, c.tab, V, &variant, &F));
+    launch_tile<false>(F, variant, c.dens, w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
                        gp, s);
     HIPCHK(hipGetLastError());
@@ -2397,7 +2476,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
     if (!replay) {
         B.reset();
         std::vector<Cond> C(nRec);
-        int64_t o_mask = 0, o_blk = 0, o_ax = 0, o_axs = 0, o_img = 0, o_aa = 0;
+        int64_t o_mask = 0, o_blk = 0, o_ax = 0, o_axs = 0, o_axq = 0, o_img = 0, o_aa = 0;
         int64_t max_scan = 1;
         for (int i = 0; i < nRec; i++) {
             Cond& c = C[i];
@@ -2417,6 +2496,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             c.o_mask = o_mask; o_mask += std::max<int64_t>(c.n, 1);
             c.o_blk = o_blk; o_blk += c.nbc + 1;
             c.o_ax = o_ax; o_ax += c.plan.Mx + c.plan.My + c.plan.Mz;
+            c.o_axq = o_axq; o_axq += c.plan.Mx + c.plan.My + 2 * (int64_t)c.plan.Mz;
             c.o_axs = o_axs; o_axs += c.plan.Ex + c.plan.Ey + c.plan.Ez + 3;
             c.o_img = o_img; o_img += c.plane;
             c.o_aa = o_aa; o_aa += c.plan.nPrec;
@@ -2428,7 +2508,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         CHK(B.axc.ensure(sizeof(int) * o_ax));
         CHK(B.axz.ensure(sizeof(double) * o_ax));
         CHK(B.axs.ensure(sizeof(int) * o_axs));
-        CHK(B.axq.ensure(sizeof(QL) * o_ax));
+        CHK(B.axq.ensure(sizeof(QL) * o_axq));
         CHK(B.img.ensure(sizeof(double) * o_img));
         CHK(B.aa.ensure(sizeof(double) * std::max<int64_t>(o_aa, 1)));
         CHK(B.guard.ensure(sizeof(unsigned long long) * nRec));
@@ -2439,16 +2519,19 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         {
             std::vector<int> hc(o_ax), hs(o_axs);
             std::vector<double> hz(o_ax), haa(std::max<int64_t>(o_aa, 1));
-            std::vector<QL> hq(o_ax);
+            std::vector<QL> hq(o_axq);
             for (int i = 0; i < nRec; i++) {
                 Cond& c = C[i];
-                int64_t oc = c.o_ax, os = c.o_axs;
+                int64_t oc = c.o_ax, os = c.o_axs, oq = c.o_axq;
+                for (int a = 0; a < 4; a++) {
+                    std::copy(c.tab.ql[a].begin(), c.tab.ql[a].end(), hq.begin() + oq);
+                    c.ax.ql[a] = B.axq.as<QL>() + oq;
+                    oq += (int64_t)c.tab.ql[a].size();
+                }
                 for (int a = 0; a < 3; a++) {
                     std::copy(c.tab.cell[a].begin(), c.tab.cell[a].end(), hc.begin() + oc);
                     std::copy(c.tab.zi[a].begin(), c.tab.zi[a].end(), hz.begin() + oc);
                     std::copy(c.tab.start[a].begin(), c.tab.start[a].end(), hs.begin() + os);
-                    std::copy(c.tab.ql[a].begin(), c.tab.ql[a].end(), hq.begin() + oc);
-                    c.ax.ql[a] = B.axq.as<QL>() + oc;
                     c.ax.cell[a] = B.axc.as<int>() + oc;
                     c.ax.zi[a] = B.axz.as<double>() + oc;
                     c.ax.start[a] = B.axs.as<int>() + os;
@@ -2460,7 +2543,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             HIPCHK(hipMemcpy(B.axc.p, hc.data(), sizeof(int) * o_ax, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(B.axz.p, hz.data(), sizeof(double) * o_ax, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(B.axs.p, hs.data(), sizeof(int) * o_axs, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.axq.p, hq.data(), sizeof(QL) * o_ax, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.axq.p, hq.data(), sizeof(QL) * o_axq, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(B.aa.p, haa.data(), sizeof(double) * haa.size(), hipMemcpyHostToDevice));
         }
         // counting pass: sizes of every condition
