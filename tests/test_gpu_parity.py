@@ -127,6 +127,28 @@ def test_field_fast_membership_exact(case):
     assert np.max(np.abs(rho - orho)) < 1e-5
 
 
+@pytest.mark.parametrize("name", ["c0_sg0.01_density", "c2_small_g200_density", "c1_small_zpad"])
+@pytest.mark.parametrize("n_prec", [8, 19, 40])
+def test_fast_integrators_agree(name, n_prec, monkeypatch):
+    """8 or more precession points per pass go through the shared RK4 step
+    matrix (passes of 32, 16, 8, then the stage-by-stage march for the rest:
+    19 = 16 + 2 + 1, 40 = 32 + 8).  It is the same map as the stage-by-stage
+    march: the two FAST images differ by FP32 rounding, and both meet the
+    FAST bar against the oracle's FP64 image."""
+    print("\n[integrators]", name, n_prec, flush=True)
+    x, im = CASES[name]()
+    ref = orc.Condition(im.rec, x, 0.005, n_prec).image(nthreads=NTH)
+    poly = im.computeD2BI_img(x, 0.005, n_prec, mode="fast")
+    monkeypatch.setenv("D2BI_FAST_INTEGRATOR", "stage")
+    stage = im.computeD2BI_img(x, 0.005, n_prec, mode="fast")
+    d = np.max(np.abs(poly - stage))
+    print("[integrators] %s P=%d |poly - stage| %.3e  poly vs fp64 %.3e  stage vs fp64 %.3e"
+          % (name, n_prec, d, np.max(np.abs(poly - ref)), np.max(np.abs(stage - ref))), flush=True)
+    assert d <= 2e-6
+    assert np.max(np.abs(poly - ref)) <= FAST_TOL
+    assert np.max(np.abs(stage - ref)) <= FAST_TOL
+
+
 @pytest.mark.parametrize("n_prec", [1, 5])
 def test_image_fp64_and_fast(case, n_prec, fast_variant):
     name, x, im, oc = case
