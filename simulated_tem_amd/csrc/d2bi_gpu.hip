@@ -1497,8 +1497,13 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
         } else {
             // ROW: steps of NQ candidates (one per part), blocks of 8 steps;
             // the guard band in u is the exponent's band (|u| ~ ln2 |t| near
-            // the threshold, so it is 1.44 times as wide)
+            // the threshold, so it is 1.44 times as wide).  The member weight
+            // is X E = u + 1, taken as w = s (u + 1) with the step
+            // s = clamp(2^40 u) in [0, 1]: 1 for u >= 2^-40, 0 for u <= 0, and
+            // fractional only inside the guard band, whose pairs the block's
+            // re-walk settles.
             const int nst = (n + NQ - 1) / NQ;
+            auto step = [](float u) { return __builtin_amdgcn_fmed3f(u * 1099511627776.0f, 0.0f, 1.0f); };
             for (int kb = 0; kb < nst; kb += 8) {
                 const int ke = min(kb + 8, nst);
                 float dmin = 3.4028234663852886e38f;
@@ -1516,14 +1521,14 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     } else {
                         sn = sA[REC * j + 2].x;
                     }
-                    const float ec = E * B.w, es = E * sn;
                     const float xv[4] = {X.x, X.y, X.z, X.w};
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const float u = fmaf(xv[i], E, -1.0f);
-                        const float xs = u >= 0.0f ? xv[i] : 0.0f;
-                        rre[i] = fmaf(xs, ec, rre[i]);
-                        rim[i] = fmaf(xs, es, rim[i]);
+                        const float s = step(u);
+                        const float w = fmaf(s, u, s);
+                        rre[i] = fmaf(w, B.w, rre[i]);
+                        rim[i] = fmaf(w, sn, rim[i]);
                         dmin = fminf(dmin, fabsf(u));
                         if (DENSITY) umax[i] = fmaxf(umax[i], u);
                     }
@@ -1536,7 +1541,6 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                         const float4 C = sA[REC * j + 2];
                         float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
                         if (ROW == 2) E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
-                        const float ec = E * B.w, es = E * C.x;
                         const float xv[4] = {X.x, X.y, X.z, X.w};
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
@@ -1546,11 +1550,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                                 if (row_act && x0 + i <= x1) nguard++;
                                 const bool in = exact_at(P.ax.ql[0][vx].lc, vx, __float_as_int(C.z),
                                                          __float_as_int(C.y));
-                                if (in != (u >= 0.0f)) {
-                                    const float d = in ? xv[i] : -xv[i];
-                                    rre[i] = fmaf(d, ec, rre[i]);
-                                    rim[i] = fmaf(d, es, rim[i]);
-                                }
+                                // replace the pass's s (u + 1) by the exact test's weight
+                                const float s = step(u);
+                                const float d = (in ? u + 1.0f : 0.0f) - fmaf(s, u, s);
+                                rre[i] = fmaf(d, B.w, rre[i]);
+                                rim[i] = fmaf(d, C.x, rim[i]);
                             }
                         }
                     }
