@@ -29,14 +29,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector/matrix peak
-PEAK_TRANS_PER_S = 157.3e12 / 16     # transcendentals: quarter of the non-packed FP32 op rate
+PEAK_TRANS_PER_S = 157.3e12 / 16     # transcendentals: quarter rate of one wave64 instruction per clock per CU
 FLOPS_PER_VOXEL = 388.0
 TRANS_PER_VOXEL = 30.0
 # HBM bytes per k_voxel_tile launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes scaled by a 1 GiB copy's counts (profiles/k_voxel_tile_c1_traffic.md);
 # a profiler cannot run inside the timed process, so the figure is recorded
 # per workload
-TRAFFIC_BYTES = {"c1": 2.963e9}
+TRAFFIC_BYTES = {"c1": 2.644e9}
 
 
 def log(*a):
