@@ -1138,7 +1138,8 @@ struct FastParams {
     int tz;                     // tile depth in voxels (4, or 8 for the 4 x 4 x 8 row kernel)
 };
 
-// One wave per 4x4x4 voxel tile.  The tile's candidates are the points of
+// One wave per voxel tile: 4x4x4 voxels, or 4x4x8 for the row kernel with
+// TZ = 8 (x, y, z; x fastest).  The tile's candidates are the points of
 // every link cell in the unwrapped range [cell(first voxel)-1,
 // cell(last voxel)+1] on each axis; an unwrapped cell u stands for cell
 // mod(u, E) displaced by u - mod(u, E) cells, which is exactly the periodic
@@ -1169,9 +1170,10 @@ struct FastParams {
 // ROW = false: lane = voxel; S = sum w (cos, sin) is accumulated in blocks of
 // 32 candidates whose sums are merged with Kahan compensation.
 //
-// ROW = true: the wave is 16 voxel rows along x (one per (y, z) of the tile)
-// times 4 candidate quarters; lane (row, s) walks candidates s, s+4, s+8, ...
-// for the four voxels of its row.  The Gaussian factorises along x:
+// ROW = true: the wave is the tile's voxel rows along x (one per (y, z): 16
+// for TZ = 4, 32 for TZ = 8) times NQ candidate parts (4 or 2); lane (row, s)
+// walks candidates s, s+NQ, s+2NQ, ... for the four voxels of its row.  The
+// Gaussian factorises along x:
 //     2^(e_i - e_rc) = X_i * 2^t,   X_i = 2^(k2 (p_x - q_x,i)^2),
 //     t = k2 ((p_y - q_y)^2 + (p_z - q_z)^2) - e_rc   (e_rc = k2 (3 sigma)^2),
 // and the four x positions q_x,i are the tile's, common to every row, so
@@ -1179,9 +1181,9 @@ struct FastParams {
 // spends one v_exp_f32 per candidate on four voxels.  Membership e_i >= e_rc
 // is u_i = X_i 2^t - 1 >= 0; a block in which some |u_i| lies within the
 // guard band is walked again and the pairs of the band are re-tested in FP64,
-// correcting the sums where the FP64 test disagrees.  The quarters' sums are
+// correcting the sums where the FP64 test disagrees.  The parts' sums are
 // added in a fixed order through LDS at the end, after which lane (row, s)
-// holds voxel s of its row: the same lane-to-voxel map as ROW = false.
+// holds voxels s NV .. s NV + NV - 1 of its row (NV = 4 / NQ).
 //
 // Thin cells.  Link cells are at least 3 sigma wide along x and y (Ex <=
 // Dx/deltaM + 2 with deltaM >= 3 sigma / min(a_x, a_y)), so along those axes
