@@ -356,6 +356,14 @@ __device__ inline bool bin_point(const BinParams& P, const double* x3, int nn, d
     return is;
 }
 
+// Replica bits set in any lane of the wave (wave-uniform).
+__device__ __forceinline__ uint32_t wave_or(uint32_t m)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m |= (uint32_t)__shfl_xor((int)m, o, 64);
+    return __builtin_amdgcn_readfirstlane(m);
+}
+
 // Pass 1: replica predicate bits per atom and per-(replica, block) counts.
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __restrict__ x, BinParams P,
                                                             uint32_t* __restrict__ mask,
@@ -406,9 +414,11 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __rest
             }
             mask[j] = m;
         }
-        for (int nn = 0; nn < P.nuvw; nn++) {
-            unsigned long long b = __ballot((m >> nn) & 1u);
-            if (lane == 0 && b) atomicAdd(&cnt[nn], (int)__popcll(b));
+        // only the replicas some lane of the wave lies in
+        for (uint32_t wm = wave_or(m); wm; wm &= wm - 1) {
+            const int nn = __builtin_ctz(wm);
+            const unsigned long long b = __ballot((m >> nn) & 1u);
+            if (lane == 0) atomicAdd(&cnt[nn], (int)__popcll(b));
         }
     }
     __syncthreads();
@@ -437,14 +447,19 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
     for (int r = 0; r < BIN_ROUNDS; r++) {
         int64_t j = (int64_t)blockIdx.x * BIN_TILE + r * BIN_BLOCK + threadIdx.x;
         uint32_t m = (j < P.n) ? mask[j] : 0u;
-        for (int nn = 0; nn < P.nuvw; nn++) {
-            unsigned long long b = __ballot((m >> nn) & 1u);
+        // the wave's count per replica: zero, then the replicas some lane lies in
+        const uint32_t wm0 = wave_or(m);
+        if (lane < P.nuvw) wcnt[wave][lane] = 0;
+        for (uint32_t wm = wm0; wm; wm &= wm - 1) {
+            const int nn = __builtin_ctz(wm);
+            const unsigned long long b = __ballot((m >> nn) & 1u);
             if (lane == 0) wcnt[wave][nn] = (int)__popcll(b);
         }
         __syncthreads();
         if (m) {
             double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-            for (int nn = 0; nn < P.nuvw; nn++) {
+            for (uint32_t wm = wm0; wm; wm &= wm - 1) {
+                const int nn = __builtin_ctz(wm);
                 unsigned long long b = __ballot((m >> nn) & 1u);   // uniform among lanes with m != 0
                 if (!((m >> nn) & 1u)) continue;
                 int off = run[nn];
