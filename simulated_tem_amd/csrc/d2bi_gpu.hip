@@ -1709,20 +1709,33 @@ __global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const v
             cgi = P.bg * f.y;
         }
     };
+    // samples of the next PD double steps are in flight: ring[2u] and
+    // ring[2u + 1] hold samples ii + 1, ii + 2 of the double step at ii in
+    // slot u (ii = i0 + 2u), reloaded with those of ii + 2 PD once used.  A
+    // column is a chain of dependent loads, so the depth, not the bandwidth,
+    // bounds a wave's time when few points are marched per sample.
+    constexpr int PD = NP <= 4 ? 4 : 1;
     float c0a = 0, cgra = 0, cgia = 0;
-    float4 fb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), fc = fb;
+    float4 ring[2 * PD];
+#pragma unroll
+    for (int k = 0; k < 2 * PD; k++) ring[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (nz >= 2) {
         coef(load(0), c0a, cgra, cgia);
-        fb = load(1);
-        fc = load(2);
+#pragma unroll
+        for (int k = 0; k < 2 * PD; k++)
+            if (k + 1 <= nz) ring[k] = load(k + 1);
     }
-    for (int ii = 0; ii <= nz - 2; ii += 2) {
+    for (int i0 = 0; i0 <= nz - 2; i0 += 2 * PD) {
+#pragma unroll
+    for (int u = 0; u < PD; u++) {
+        const int ii = i0 + 2 * u;
+        if (ii > nz - 2) break;
         float c0b, cgrb, cgib, c0c, cgrc, cgic;
-        coef(fb, c0b, cgrb, cgib);
-        coef(fc, c0c, cgrc, cgic);
-        if (ii + 4 <= nz) {
-            fb = load(ii + 3);
-            fc = load(ii + 4);
+        coef(ring[2 * u], c0b, cgrb, cgib);
+        coef(ring[2 * u + 1], c0c, cgrc, cgic);
+        if (ii + 2 * PD + 2 <= nz) {
+            ring[2 * u] = load(ii + 2 * PD + 1);
+            ring[2 * u + 1] = load(ii + 2 * PD + 2);
         }
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -1758,6 +1771,7 @@ __global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const v
             pgr[p] = fmaf(dgr, dz3, agr); pgi[p] = fmaf(dgi, dz3, agi);
         }
         c0a = c0c; cgra = cgrc; cgia = cgic;
+    }
     }
     double acc = 0.0;
 #pragma unroll
