@@ -2144,6 +2144,21 @@ static void fill_vox_params(const d2bi_record* r, const d2bi_plan* p, const Axis
 // 4 x 4 x 4 tiles) use 4 voxels, 2 (rows of 4 x 4 x 8 tiles) 8.
 static int variant_tz(int variant) { return variant == 2 ? 8 : 4; }
 
+// Link cells in the largest unwrapped cell range of a 4 x 4 x tz tile.
+static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz)
+{
+    int worst = 1;
+    for (int a = 0; a < 3; a++) {
+        const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
+        const int ts = a == 2 ? tz : TILE;
+        int mx = 0;
+        for (int t0 = 0; t0 < M; t0 += ts)
+            mx = std::max(mx, tab.cell[a][std::min(M, t0 + ts) - 1] - tab.cell[a][t0] + 3);
+        worst *= mx;
+    }
+    return worst;
+}
+
 // Launch parameters of the FAST voxel kernel for a plan, tiles of 4 x 4 x tz.
 static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo, int tz)
 {
@@ -2174,15 +2189,7 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
                       p->a_col[4], rc);
     F.thin_z = p->a_col[8] < thin ? 1 : 0;
     // the unwrapped cell range of a tile must fit the per-tile cell table
-    int worst = 1;
-    for (int a = 0; a < 3; a++) {
-        const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
-        const int ts = a == 2 ? tz : TILE;
-        int mx = 0;
-        for (int t0 = 0; t0 < M; t0 += ts)
-            mx = std::max(mx, tab.cell[a][std::min(M, t0 + ts) - 1] - tab.cell[a][t0] + 3);
-        worst *= mx;
-    }
+    const int worst = tile_cells(p, tab, tz);
     if (worst > TILE_MAXC) return seterr(D2BI_ERR_CAPACITY, "voxel tile spans %d link cells (max %d)", worst, TILE_MAXC);
     return D2BI_OK;
 }
@@ -2193,11 +2200,8 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
 static int variant_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, int* variant,
                           FastParams* F)
 {
-    if (*variant == 2) {
-        if (fast_params(p, tab, V, F, 8) == D2BI_OK) return D2BI_OK;
-        *variant = 1;
-    }
-    return fast_params(p, tab, V, F, 4);
+    if (*variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) *variant = 1;
+    return fast_params(p, tab, V, F, variant_tz(*variant));
 }
 
 // Kernel instance for (diagnostics, density, variant); link cells thinner
@@ -2339,8 +2343,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     };
     const int variant = fast_variant(p, dens, want_diag, F.thin_z != 0, [&](int v) -> float {
         // a variant whose tiles span too many link cells is not timed
-        FastParams Fv;
-        if (fast_params(p, tab, V, &Fv, variant_tz(v)) < 0) return 3.4e38f;
+        if (tile_cells(p, tab, variant_tz(v)) > TILE_MAXC) return 3.4e38f;
         HIPCHK(hipEventRecord(g.ev[7], s));
         CHK(run_variant(v));
         HIPCHK(hipEventRecord(g.ev[8], s));
