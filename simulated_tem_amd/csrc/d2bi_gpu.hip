@@ -837,7 +837,7 @@ static int bin_params(const d2bi_record* r, const d2bi_plan* p, BinParams* P)
 //   cell = mod(floor(zi) + 16*Ex, Ex), zi = zi - cell
 // (neighbourList1b, Lib_LinkCell3D.f90:1549-1565).  The zero off-diagonal
 // terms of the reference's 3x3 products add +-0 and change no bit.
-static constexpr int TILE = 4;                       // FAST voxel tiles: 4x4x4 voxels = one wave64
+static constexpr int TILE = 4;                       // FAST voxel tiles: 4 voxels along x and y (z: 4 or 8)
 
 // FP32 voxel position along one axis relative to the link cell of its
 // FAST tile's first voxel, and that cell offset
@@ -1378,7 +1378,8 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     float emax = -3.4028234663852886e38f;
     int cnt = 0;
     unsigned nguard = 0;
-    // ROW: sums and largest u of the row's four voxels over this quarter
+    // ROW: sums and largest u of the row's four voxels over this lane's part
+    // of the candidates
     float rre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float umax[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
 
