@@ -66,7 +66,9 @@ def cpu_baseline(im, x, sample_cols, nthreads):
 def batch_bench(args, imaging, synth):
     """Batched conditions through the graph-captured batch API: every
     condition re-binned and re-voxelised, P precession points integrated per
-    voxel field.  Plain FP64 = eager per-condition calls on a subset."""
+    voxel field, the images left in HBM (value) or copied to host memory
+    (value_image_to_host, PCIe included).  Plain FP64 = eager per-condition
+    calls on a subset."""
     import numpy as np
     cfg = dict(synth.CONFIGS[args.batch])
     log("[bench] batch workload", args.batch)
@@ -78,16 +80,22 @@ def batch_bench(args, imaging, synth):
     for im, pa, npr in conds:
         p = im.plan(len(x), pa, npr)
         imaged += p.Mx * p.My * p.Mz * npr
+    dev = imaging.DeviceBuffer((imaging.batch_image_size(batch),))
     t0 = time.perf_counter()
-    imgs = imaging.computeD2BI_img_batch(batch, mode="fast")          # plans, tunes, captures
+    imaging.computeD2BI_img_batch(batch, mode="fast", out=dev)       # plans, tunes, captures
     log("[bench] batch first call (capture) %.2f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
     for _ in range(args.batch_steps):
-        imgs = imaging.computeD2BI_img_batch(batch, mode="fast")
+        imaging.computeD2BI_img_batch(batch, mode="fast", out=dev)
     t1 = time.perf_counter()
     s_per_batch = (t1 - t0) / args.batch_steps
     value = imaged / s_per_batch
     log("[bench] batch fast: %.3f s/batch, %.3e imaged voxels/s" % (s_per_batch, value))
+    t0 = time.perf_counter()
+    imgs = imaging.computeD2BI_img_batch(batch, mode="fast")
+    s_host = time.perf_counter() - t0
+    assert np.array_equal(dev.numpy(), np.concatenate([i.ravel() for i in imgs]))
+    log("[bench] batch fast, images copied to host: %.3f s/batch" % s_host)
     idx = np.linspace(0, len(conds) - 1, max(1, args.batch_fp64)).round().astype(int)
     errs, imaged64 = [], 0
     t64 = 0.0
@@ -105,6 +113,7 @@ def batch_bench(args, imaging, synth):
     return {"workload": args.batch, "conditions": len(conds), "n_prec": conds[0][2],
             "imaged_voxels_per_batch": imaged, "value": value, "unit": "imaged voxels/s",
             "s_per_batch": s_per_batch, "steps": args.batch_steps,
+            "value_image_to_host": imaged / s_host,
             "fp64_value": fp64_value, "fp64_conditions": len(idx),
             "fast_over_fp64": value / fp64_value, "max_abs_err_vs_fp64": max(errs)}
 

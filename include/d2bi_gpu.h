@@ -236,6 +236,14 @@ int d2bi_gpu_image_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int32
                          const double* precAngle, const int32_t* nPrec, int32_t mode,
                          double* img, d2bi_stats* stats);
 
+/* Same, with the images left in HBM: d_img is a device pointer to the sum of
+ * Mx*My doubles over the records, for GPU-side consumers and for timing the
+ * batch without the PCIe copy of its images.  Returns when the images are
+ * complete. */
+int d2bi_gpu_image_batch_dev_out(const d2bi_record* recs, const d2bi_atoms* atoms, int32_t nRec,
+                                 const double* precAngle, const int32_t* nPrec, int32_t mode,
+                                 double* d_img, d2bi_stats* stats);
+
 /* Voxel field for tests: per voxel (x fastest, then y, then z; Mz planes)
  * the normalised phase factor (re,im), density rho, neighbour count and the
  * minimum squared range.  Any output pointer may be null.  In FAST mode with

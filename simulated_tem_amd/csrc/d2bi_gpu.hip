@@ -2731,8 +2731,9 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tm
 // (counting, binning, voxelisation, integration of every condition) into one
 // HIP graph.  Later calls with the same inputs and an unchanged workspace
 // replay the graph: every replay recomputes everything from the atoms.
+// img_out: host memory, or device memory when dev_out.
 static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec, const double* precAngle,
-                     const int32_t* nPrec, int mode, double* img_host, d2bi_stats* st)
+                     const int32_t* nPrec, int mode, double* img_out, bool dev_out, d2bi_stats* st)
 {
     if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
     Batch& B = g_batch;
@@ -2902,7 +2903,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
     HIPCHK(hipEventRecord(g.ev[0], s));
     HIPCHK(hipGraphLaunch(B.exec, s));
     HIPCHK(hipEventRecord(g.ev[1], s));
-    HIPCHK(hipMemcpyAsync(img_host, B.img.p, sizeof(double) * B.total_plane, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(img_out, B.img.p, sizeof(double) * B.total_plane,
+                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (st) {
         std::vector<unsigned long long> gcv(nRec);
@@ -3070,7 +3072,27 @@ int d2bi_gpu_image_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int32
     for (int i = 0; i < nRec; i++)
         if (!atoms[i]) return seterr(D2BI_ERR_ARG, "null atom set %d", i);
     if (nRec == 0) return D2BI_OK;
-    return run_batch(recs, atoms, nRec, precAngle, nPrec, mode, img, stats);
+    return run_batch(recs, atoms, nRec, precAngle, nPrec, mode, img, false, stats);
+}
+
+int d2bi_gpu_image_batch_dev_out(const d2bi_record* recs, const d2bi_atoms* atoms, int32_t nRec,
+                                 const double* precAngle, const int32_t* nPrec, int32_t mode, double* d_img,
+                                 d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (nRec < 0 || (nRec > 0 && (!recs || !atoms || !precAngle || !nPrec || !d_img)))
+        return seterr(D2BI_ERR_ARG, "null argument");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    for (int i = 0; i < nRec; i++)
+        if (!atoms[i]) return seterr(D2BI_ERR_ARG, "null atom set %d", i);
+    if (nRec == 0) return D2BI_OK;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, d_img) != hipSuccess) {
+        (void)hipGetLastError();                 // a failed query must not linger as the last error
+        return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    }
+    if (attr.type != hipMemoryTypeDevice) return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    return run_batch(recs, atoms, nRec, precAngle, nPrec, mode, d_img, true, stats);
 }
 
 int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t nAtoms, int32_t mode,

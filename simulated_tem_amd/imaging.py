@@ -315,9 +315,20 @@ class DynamicalTwoBeamImaging:
         return cs, ids[:k], frac[:3 * k].reshape(k, 3), gd[:k]
 
 
-def computeD2BI_img_batch(conditions, mode="fast", stats=None):
+def batch_image_size(conditions):
+    """Doubles the images of a batch occupy (sum of Mx*My)."""
+    total = 0
+    for c in conditions:
+        p = c[0].plan(1)
+        total += p.Mx * p.My
+    return total
+
+
+def computeD2BI_img_batch(conditions, mode="fast", stats=None, out=None):
     """Batched conditions: list of (DynamicalTwoBeamImaging, Atoms, precAngle,
-    nPrec).  Returns the list of images."""
+    nPrec).  Returns the list of images; with out (a DeviceBuffer of
+    batch_image_size(conditions) doubles) the images stay in HBM, one after
+    another, and out is returned."""
     _ensure()
     n = len(conditions)
     recs = (D2biRecord * n)(*[c[0].rec for c in conditions])
@@ -328,8 +339,15 @@ def computeD2BI_img_batch(conditions, mode="fast", stats=None):
     for c in conditions:
         p = c[0].plan(1)
         shapes.append((p.My, p.Mx))
-    img = np.zeros(sum(a * b for a, b in shapes))
     st = stats if stats is not None else D2biStats()
+    if out is not None:
+        total = sum(a * b for a, b in shapes)
+        if not isinstance(out, DeviceBuffer) or int(np.prod(out.shape)) != total:
+            raise ValueError("out must be a DeviceBuffer of %d doubles" % total)
+        check(abi.load().d2bi_gpu_image_batch_dev_out(recs, hs, n, ptr(pa), iptr(npr), MODES[mode],
+                                                      C.c_void_p(out.ptr), C.byref(st)))
+        return out
+    img = np.zeros(sum(a * b for a, b in shapes))
     check(abi.load().d2bi_gpu_image_batch(recs, hs, n, ptr(pa), iptr(npr), MODES[mode], ptr(img),
                                           C.byref(st)))
     out, off = [], 0

@@ -8,6 +8,7 @@ Bars (per the imaging contract):
   FAST image                                    max |dI| <= 1e-4
   blur                                          max |d| <= 1e-15
 """
+import ctypes as C
 import json
 import math
 import os
@@ -16,7 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from simulated_tem_amd import imaging, synth
+from simulated_tem_amd import abi, imaging, synth
 from simulated_tem_amd.imaging import DynamicalTwoBeamImaging, input_supercell, lattice_a0
 
 pytestmark = pytest.mark.gpu
@@ -313,6 +314,22 @@ def test_batch_graph_replays_equal_eager(mode):
     ref = orc.Condition(im3.rec, x3, 0.004, 6).image()
     tol = FP64_TOL if mode == "fp64" else FAST_TOL
     assert np.max(np.abs(out[2] - ref)) <= tol
+    # the device-output form: the same images, left in HBM one after another
+    dev = imaging.DeviceBuffer((imaging.batch_image_size(conds),))
+    for rep in range(2):
+        assert imaging.computeD2BI_img_batch(conds, mode=mode, out=dev) is dev
+        assert np.array_equal(dev.numpy(), np.concatenate([e.ravel() for e in eager])), rep
+    with pytest.raises(ValueError):
+        imaging.computeD2BI_img_batch(conds, mode=mode, out=imaging.DeviceBuffer((7,)))
+    host = np.zeros(imaging.batch_image_size(conds))
+    recs = (imaging.D2biRecord * 3)(*[c[0].rec for c in conds])
+    hs = (C.c_void_p * 3)(*[c[1].h for c in conds])
+    pa = np.array([c[2] for c in conds])
+    npr = np.array([c[3] for c in conds], dtype=np.int32)
+    rc = abi.load().d2bi_gpu_image_batch_dev_out(recs, hs, 3, pa.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 npr.ctypes.data_as(C.POINTER(C.c_int32)), imaging.MODES[mode],
+                                                 C.c_void_p(host.ctypes.data), None)
+    assert rc != 0                                   # host memory is refused
 
 
 def test_uneven_batch_c4_style():
