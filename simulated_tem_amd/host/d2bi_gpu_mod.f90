@@ -49,7 +49,7 @@ module d2bi_gpu_mod
     public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_plan_image
     public :: d2bi_setup_tilt_sg, d2bi_setup_tilt_ur, d2bi_setup_tilt_ng, d2bi_rotate_to_standard_frame
     public :: d2bi_gpu_atoms_upload, d2bi_gpu_atoms_free, d2bi_gpu_image_dev, d2bi_gpu_blur
-    public :: d2bi_gpu_image_aperture, d2bi_gpu_image_batch
+    public :: d2bi_gpu_image_aperture, d2bi_gpu_image_batch, d2bi_gpu_image_batch_dev_out
     public :: d2bi_write_png16, d2bi_error_message
 
     interface
@@ -147,6 +147,18 @@ module d2bi_gpu_mod
             real(c_double), intent(in)      :: precAngle(*)
             integer(c_int32_t), intent(in)  :: nPrec(*)
             real(c_double), intent(out)     :: img(*)
+            type(d2bi_stats), intent(out)   :: stats
+        end function
+        ! the images stay in HBM: d_img is a device address (e.g. from hipMalloc)
+        integer(c_int) function d2bi_gpu_image_batch_dev_out(recs, atoms, nRec, precAngle, nPrec, mode, d_img, &
+                stats) bind(c, name="d2bi_gpu_image_batch_dev_out")
+            import :: c_int, c_int32_t, c_double, c_ptr, d2bi_record, d2bi_stats
+            type(d2bi_record), intent(in)   :: recs(*)
+            type(c_ptr), intent(in)         :: atoms(*)
+            integer(c_int32_t), value       :: nRec, mode
+            real(c_double), intent(in)      :: precAngle(*)
+            integer(c_int32_t), intent(in)  :: nPrec(*)
+            type(c_ptr), value              :: d_img
             type(d2bi_stats), intent(out)   :: stats
         end function
         integer(c_int) function d2bi_gpu_blur(fin, nx, ny, t, fout) bind(c, name="d2bi_gpu_blur")
