@@ -50,6 +50,10 @@ CASES = {
     "c2_small_g200_density": lambda: _cond("c2", cells=(14, 12, 30), sg=-0.02),
     "c0_k111_g1m10": lambda: _cond("c0", cells=(9, 11, 7), g=(1, -1, 0), k=(1, 1, 1),
                                    d=(25.0, 27.0, 19.0)),
+    # a 33 x 31 x 29 voxel grid: the last tile along every axis is partial
+    # (4-voxel tiles in x and y, 4- and 8-voxel tiles in z)
+    "c0_k111_ragged": lambda: _cond("c0", cells=(10, 9, 8), g=(1, -1, 0), k=(1, 1, 1),
+                                    d=(26.1, 24.7, 22.9)),
     # link cells thinner than 3 sigma along z (4.725 A, Ez = 8), and a foil
     # of three cells (Ez = 3, 2.63 A): points two cells away in z can lie
     # within 3 sigma and neighbourList1b leaves them out
