@@ -236,9 +236,10 @@ int d2bi_gpu_image_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int32
                          const double* precAngle, const int32_t* nPrec, int32_t mode,
                          double* img, d2bi_stats* stats);
 
-/* Same, with the images left in HBM: d_img is a device pointer to the sum of
- * Mx*My doubles over the records, for GPU-side consumers and for timing the
- * batch without the PCIe copy of its images.  Returns when the images are
+/* The same batch (the d2bi driver's condition loop, d2bi.f90:1352-1394) with
+ * the images left in HBM: d_img is a device pointer to the sum of Mx*My
+ * doubles over the records, for GPU-side consumers and for timing the batch
+ * without the PCIe copy of its images.  Returns when the images are
  * complete. */
 int d2bi_gpu_image_batch_dev_out(const d2bi_record* recs, const d2bi_atoms* atoms, int32_t nRec,
                                  const double* precAngle, const int32_t* nPrec, int32_t mode,
