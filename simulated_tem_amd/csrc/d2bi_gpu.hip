@@ -1694,8 +1694,15 @@ __global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const v
     // sample Mz is sample 0.  Samples are loaded one double-step ahead.
     auto load = [&](int k) -> float4 {
         const int64_t o = col + plane * (k == nz ? 0 : k);
-        if (DENS) return static_cast<const float4*>(field)[o];
-        const float2 f = static_cast<const float2*>(field)[o];
+        // each sample is read once: non-temporal loads keep the stream from
+        // displacing cached data
+        typedef float ntf2 __attribute__((ext_vector_type(2)));
+        typedef float ntf4 __attribute__((ext_vector_type(4)));
+        if (DENS) {
+            const ntf4 v = __builtin_nontemporal_load(static_cast<const ntf4*>(field) + o);
+            return make_float4(v.x, v.y, v.z, v.w);
+        }
+        const ntf2 f = __builtin_nontemporal_load(static_cast<const ntf2*>(field) + o);
         return make_float4(f.x, f.y, 1.0f, 0.0f);
     };
     auto coef = [&](const float4& f, float& c0, float& cgr, float& cgi) {
