@@ -1,4 +1,6 @@
-// d2bi_gpu.hip -- gfx950 kernels and the device half of the C-ABI.
+// d2bi_gpu.hip -- gfx950 kernels and the device half of the C-ABI: one
+// translation unit made of the parts included below (d2bi_*.cuh) and the
+// C-ABI entry points at its end.
 //
 // Stages of one imaging condition (one "field"):
 //   (a) binning      findGdotR + LinkCell3D add as a stable counting sort of
@@ -12,8 +14,8 @@
 // of every candidate whose FP32 range lies in a guard band around 3 sigma,
 // so voxel membership stays identical to FP64.
 //
-// This file is compiled with -ffp-contract=off: every fused multiply-add in
-// it is written explicitly (fmaf) and only appears in FAST-mode FP32 code.
+// The unit is compiled with -ffp-contract=off: every fused multiply-add in it
+// is written explicitly (fmaf) and only appears in FAST-mode FP32 code.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,2907 +32,16 @@
 #include "../../include/d2bi_gpu.h"
 #include "d2bi_internal.h"
 
-namespace d2bi {
-
-// ------------------------------------------------------------------ errors
-
-static char g_err[1024] = "";
-
-int seterr(int code, const char* fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-#define HIPCHK(call)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (call);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return seterr(D2BI_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                          __FILE__, __LINE__);                                           \
-    } while (0)
-
-#define CHK(call)                        \
-    do {                                 \
-        int r_ = (call);                 \
-        if (r_ != D2BI_OK) return r_;    \
-    } while (0)
-
-// ------------------------------------------------------------------ device buffers
-
-static uint64_t g_alloc_gen = 0;         // bumped on every device reallocation
-
-struct Buf {
-    void* p = nullptr;
-    size_t cap = 0;
-    int ensure(size_t bytes)
-    {
-        if (bytes <= cap && p) return D2BI_OK;
-        g_alloc_gen++;
-        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
-        size_t want = std::max<size_t>(bytes, 256);
-        want = want + want / 8;                      // headroom for slightly larger conditions
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return seterr(D2BI_ERR_HIP, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
-        }
-        cap = want;
-        return D2BI_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-struct Workspace {
-    // binning
-    Buf x;                      // atoms (temporary upload)
-    Buf mask, blk_cnt, blk_off;
-    Buf pt_cell, pt_slot, pt_frac, pt_gdotr;
-    Buf pt_p32, pt_sn32;        // FAST records in point order (FAST imaging)
-    Buf cell_cnt, cell_start, cell_ids;
-    Buf slot_id, slot_frac, slot_cs64, slot_p32, slot_sn32;
-    Buf scan_tmp;
-    Buf blk_max;                // per-block largest cell count (link-cell sort)
-    Buf guard_part;             // FAST guard re-test counters, one per 128-byte line
-    Buf counters;               // misc device counters (guard count, max per cell)
-    // voxel axis tables
-    Buf ax_cell, ax_zi, ax_start, ax_ql;
-    // fields and image
-    Buf f64_re, f64_im, f64_rho, f32_field;
-    Buf cnt, r2m;
-    Buf img;
-    Buf blur_in, blur_tmp, blur_out, blur_w, blur_k;
-    // pinned host staging
-    void* pinned = nullptr;
-    size_t pinned_cap = 0;
-    void release_all()
-    {
-        Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_p32, &pt_sn32,
-                      &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
-                      &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
-                      &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
-                      &blur_out, &blur_w, &blur_k};
-        for (Buf* b : all) b->release();
-        if (pinned) (void)hipHostFree(pinned);
-        pinned = nullptr;
-        pinned_cap = 0;
-    }
-};
-
-struct State {
-    int device = -1;
-    bool init = false;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[10] = {};
-    bool pinned_busy = false;       // ev[9] marks the last copy out of the pinned staging buffer
-    Workspace ws;
-};
-
-static State g;
-
-static int need_init()
-{
-    if (!g.init) return seterr(D2BI_ERR_NOINIT, "d2bi_gpu_init has not been called");
-    return D2BI_OK;
-}
-
-// The pinned staging buffer: a caller that leaves an asynchronous copy from
-// it in flight records ev[9] and sets pinned_busy; the next user waits for
-// that copy before writing into (or replacing) the buffer.
-static int pinned(size_t bytes, void** out)
-{
-    if (g.pinned_busy) {
-        HIPCHK(hipEventSynchronize(g.ev[9]));
-        g.pinned_busy = false;
-    }
-    if (bytes > g.ws.pinned_cap) {
-        if (g.ws.pinned) (void)hipHostFree(g.ws.pinned);
-        g.ws.pinned = nullptr;
-        g.ws.pinned_cap = 0;
-        HIPCHK(hipHostMalloc(&g.ws.pinned, bytes, hipHostMallocDefault));
-        g.ws.pinned_cap = bytes;
-    }
-    *out = g.ws.pinned;
-    return D2BI_OK;
-}
-
-// ------------------------------------------------------------------ device atoms
-
-}  // namespace d2bi
-
-struct d2bi_atoms_s {
-    double* x = nullptr;        // 3*n doubles, AoS
-    int64_t n = 0;
-};
-
-namespace d2bi {
-
-// ------------------------------------------------------------------ generic scan
-
-static constexpr int SCAN_BLOCK = 256;
-static constexpr int SCAN_ITEMS = 4;
-static constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
-
-// Block-wide exclusive scan of one int per thread (wave64 shuffles + LDS).
-__device__ inline int block_exclusive_scan(int v, int* total, int* lds /*[SCAN_BLOCK/64 + 1]*/)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) lds[wave] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) { int t = lds[w]; lds[w] = s; s += t; }
-        lds[blockDim.x >> 6] = s;
-    }
-    __syncthreads();
-    int r = incl - v + lds[wave];
-    *total = lds[blockDim.x >> 6];
-    __syncthreads();
-    return r;
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const int* __restrict__ in, int64_t n,
-                                                            int* __restrict__ sums)
-{
-    __shared__ int lds[SCAN_BLOCK / 64 + 1];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++)
-        if (base + i < n) s += in[base + i];
-    int tot;
-    block_exclusive_scan(s, &tot, lds);
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-
-// Single-block exclusive scan of the block sums (any length), in place.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int* __restrict__ sums, int64_t nb,
-                                                          int* __restrict__ total_out)
-{
-    __shared__ int lds[SCAN_BLOCK / 64 + 1];
-    int carry = 0;
-    for (int64_t b = 0; b < nb; b += SCAN_BLOCK) {
-        int64_t i = b + threadIdx.x;
-        int v = i < nb ? sums[i] : 0;
-        int tot;
-        int e = block_exclusive_scan(v, &tot, lds);
-        if (i < nb) sums[i] = e + carry;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) *total_out = carry;
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const int* __restrict__ in, int64_t n,
-                                                           const int* __restrict__ sums,
-                                                           int* __restrict__ out)
-{
-    __shared__ int lds[SCAN_BLOCK / 64 + 1];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    int v[SCAN_ITEMS];
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        v[i] = base + i < n ? in[base + i] : 0;
-        s += v[i];
-    }
-    int tot;
-    int e = block_exclusive_scan(s, &tot, lds) + sums[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        if (base + i < n) out[base + i] = e;
-        e += v[i];
-    }
-}
-
-static int64_t scan_tmp_ints(int64_t n)
-{
-    return std::max<int64_t>(1, (n + SCAN_TILE - 1) / SCAN_TILE) + 1;
-}
-
-// out[0..n-1] = exclusive scan of in, out[n] = total.  in may equal out.
-// sums: scan_tmp_ints(n) ints of scratch (no allocation here, so the call can
-// be captured into a graph).
-static int scan_exclusive_tmp(const int* in, int* out, int64_t n, int* sums, hipStream_t s)
-{
-    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb, out + n);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out);
-    HIPCHK(hipGetLastError());
-    return D2BI_OK;
-}
-
-static int scan_exclusive(const int* in, int* out, int64_t n, hipStream_t s)
-{
-    CHK(g.ws.scan_tmp.ensure(sizeof(int) * scan_tmp_ints(n)));
-    return scan_exclusive_tmp(in, out, n, g.ws.scan_tmp.as<int>(), s);
-}
-
-// ------------------------------------------------------------------ (a) binning
-
-static constexpr int BIN_BLOCK = 256;
-static constexpr int BIN_ROUNDS = 4;
-static constexpr int BIN_TILE = BIN_BLOCK * BIN_ROUNDS;
-static constexpr int MAX_REPL = 32;
-
-struct BinParams {
-    double RU[9];               // column-major
-    double delta[3];
-    double lc_delta[3];
-    double ia[9];               // link-cell inverse (column-major)
-    double gg[3];
-    double Ruvw[3 * MAX_REPL];
-    int nuvw;
-    int Ex, Ey, Ez;
-    int64_t n;                  // atoms
-    int generic;                // 1: LinkCell3D add1 (no transform, no filter)
-    int fast;                   // FAST imaging: FP32 records instead of g.r and the FP64 slot arrays
-    double cell_a[3];           // diagonal of the link-cell matrix (FAST-mode records)
-    // replica pre-classification: c ~= M x + off[nn] approximates the
-    // link-cell coordinates of replica nn (products folded, so a few
-    // roundings away from the reference's order); a replica is decided
-    // from it unless a coordinate lies within eps of a box face
-    int prefilter;
-    double M[9];                // ia . RU (column-major)
-    double off[3 * MAX_REPL];   // ia . (RU . Ruvw[nn] + delta - lc_delta)
-    double mabs, oabs;          // magnitude bounds: per unit |x|, and the rest
-};
-
-// Link-cell index of a reduced coordinate: mod(floor(y) + 16*N, N)
-// (whichCell, Lib_SimpleSupercells.f90:371-373).  For points further than
-// 16 boxes away the reference's expression goes negative; here the floor is
-// clamped and the modulo kept non-negative, which changes nothing for
-// points within range and keeps every index inside the cell array.
-__device__ inline int cell_of(double fl, int n)
-{
-    const double c = fmin(fmax(fl, -1.0e9), 1.0e9);
-    const int r = ((int)c + 16 * n) % n;
-    return r < 0 ? r + n : r;
-}
-
-// Transform atom j for replica nn: yy (rotated lab position), zz (link-cell
-// frame), and whether it lies in the link-cell box.  Operation order of
-// findGdotR (Lib_DynamicalTwoBeamImaging.f90:1225-1240) and pointInSupercell0
-// (Lib_SimpleSupercells.f90:1077-1084).
-__device__ inline bool bin_point(const BinParams& P, const double* x3, int nn, double yy[3],
-                                 double cyy[3])
-{
-    if (P.generic) {
-        yy[0] = x3[0]; yy[1] = x3[1]; yy[2] = x3[2];
-        for (int i = 0; i < 3; i++)
-            cyy[i] = (P.ia[i] * yy[0] + P.ia[3 + i] * yy[1]) + P.ia[6 + i] * yy[2];
-        return true;
-    }
-    double y0[3], zz[3];
-    for (int i = 0; i < 3; i++) y0[i] = x3[i] + P.Ruvw[3 * nn + i];
-    for (int i = 0; i < 3; i++) yy[i] = (P.RU[i] * y0[0] + P.RU[3 + i] * y0[1]) + P.RU[6 + i] * y0[2];
-    for (int i = 0; i < 3; i++) {
-        yy[i] = yy[i] + P.delta[i];
-        zz[i] = yy[i] - P.lc_delta[i];
-    }
-    for (int i = 0; i < 3; i++)
-        cyy[i] = (P.ia[i] * zz[0] + P.ia[3 + i] * zz[1]) + P.ia[6 + i] * zz[2];
-    // |floor| beyond 2^30 cannot be inside the box; clamping keeps int conversion defined
-    int ix = (int)fmin(fmax(floor(cyy[0]), -1.0e9), 1.0e9);
-    int iy = (int)fmin(fmax(floor(cyy[1]), -1.0e9), 1.0e9);
-    int iz = (int)fmin(fmax(floor(cyy[2]), -1.0e9), 1.0e9);
-    bool is = (ix == (ix + 256 * P.Ex) % P.Ex);
-    is = is && (iy == (iy + 256 * P.Ey) % P.Ey);
-    is = is && (iz == (iz + 256 * P.Ez) % P.Ez);
-    return is;
-}
-
-// Replica bits set in any lane of the wave (wave-uniform).
-__device__ __forceinline__ uint32_t wave_or(uint32_t m)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m |= (uint32_t)__shfl_xor((int)m, o, 64);
-    return __builtin_amdgcn_readfirstlane(m);
-}
-
-// Pass 1: replica predicate bits per atom and per-(replica, block) counts.
-__global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __restrict__ x, BinParams P,
-                                                            uint32_t* __restrict__ mask,
-                                                            int* __restrict__ blk_cnt, int nblk)
-{
-    __shared__ int cnt[MAX_REPL];
-    if (threadIdx.x < MAX_REPL) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    for (int r = 0; r < BIN_ROUNDS; r++) {
-        int64_t j = (int64_t)blockIdx.x * BIN_TILE + r * BIN_BLOCK + threadIdx.x;
-        uint32_t m = 0;
-        if (j < P.n) {
-            double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-            if (P.prefilter) {
-                double cb[3];
-                for (int i = 0; i < 3; i++) cb[i] = P.M[i] * x3[0] + P.M[3 + i] * x3[1] + P.M[6 + i] * x3[2];
-                // |approximate - exact| is a few units of roundoff (1e-16) of
-                // the magnitudes involved; eps is 1e-10 of their bound
-                const double eps =
-                    1.0e-10 * (1.0 + P.mabs * (fabs(x3[0]) + fabs(x3[1]) + fabs(x3[2])) + P.oabs);
-                // c in [0, E) per axis is |c - E/2| < E/2: certainly outside
-                // when |c - E/2| >= E/2 + eps, certainly inside when <= E/2 - eps
-                const double h[3] = {0.5 * P.Ex, 0.5 * P.Ey, 0.5 * P.Ez};
-                double ch[3], hout[3], hin[3];
-                for (int i = 0; i < 3; i++) {
-                    ch[i] = cb[i] - h[i];
-                    hout[i] = h[i] + eps;
-                    hin[i] = h[i] - eps;
-                }
-                for (int nn = 0; nn < P.nuvw; nn++) {
-                    bool out = false, sure = true;
-                    for (int i = 0; i < 3; i++) {
-                        const double u = fabs(ch[i] + P.off[3 * nn + i]);
-                        out = out || u >= hout[i];
-                        sure = sure && u <= hin[i];
-                    }
-                    if (out) continue;                       // outside on some axis for certain
-                    if (sure) { m |= 1u << nn; continue; }   // inside on every axis for certain
-                    double yy[3], cyy[3];
-                    if (bin_point(P, x3, nn, yy, cyy)) m |= 1u << nn;
-                }
-            } else {
-                for (int nn = 0; nn < P.nuvw; nn++) {
-                    double yy[3], cyy[3];
-                    if (bin_point(P, x3, nn, yy, cyy)) m |= 1u << nn;
-                }
-            }
-            mask[j] = m;
-        }
-        // only the replicas some lane of the wave lies in
-        for (uint32_t wm = wave_or(m); wm; wm &= wm - 1) {
-            const int nn = __builtin_ctz(wm);
-            const unsigned long long b = __ballot((m >> nn) & 1u);
-            if (lane == 0) atomicAdd(&cnt[nn], (int)__popcll(b));
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < P.nuvw) blk_cnt[(int64_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
-}
-
-// Pass 2: point number ii in (replica, atom) order, link cell, offset in
-// cell and g.r; per-cell arrival slot from an atomic counter.
-__global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restrict__ x, BinParams P,
-                                                           const uint32_t* __restrict__ mask,
-                                                           const int* __restrict__ blk_off, int nblk,
-                                                           int* __restrict__ pt_cell,
-                                                           int* __restrict__ pt_slot,
-                                                           double* __restrict__ pt_frac,
-                                                           double* __restrict__ pt_gdotr,
-                                                           float4* __restrict__ pt_p32,
-                                                           float* __restrict__ pt_sn32,
-                                                           int* __restrict__ cell_cnt)
-{
-    __shared__ int run[MAX_REPL];
-    __shared__ int wcnt[BIN_BLOCK / 64][MAX_REPL];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < P.nuvw) run[threadIdx.x] = blk_off[(int64_t)threadIdx.x * nblk + blockIdx.x];
-    __syncthreads();
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int r = 0; r < BIN_ROUNDS; r++) {
-        int64_t j = (int64_t)blockIdx.x * BIN_TILE + r * BIN_BLOCK + threadIdx.x;
-        uint32_t m = (j < P.n) ? mask[j] : 0u;
-        // the wave's count per replica: zero, then the replicas some lane lies in
-        const uint32_t wm0 = wave_or(m);
-        if (lane < P.nuvw) wcnt[wave][lane] = 0;
-        for (uint32_t wm = wm0; wm; wm &= wm - 1) {
-            const int nn = __builtin_ctz(wm);
-            const unsigned long long b = __ballot((m >> nn) & 1u);
-            if (lane == 0) wcnt[wave][nn] = (int)__popcll(b);
-        }
-        __syncthreads();
-        if (m) {
-            double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-            for (uint32_t wm = wm0; wm; wm &= wm - 1) {
-                const int nn = __builtin_ctz(wm);
-                unsigned long long b = __ballot((m >> nn) & 1u);   // uniform among lanes with m != 0
-                if (!((m >> nn) & 1u)) continue;
-                int off = run[nn];
-                for (int w = 0; w < wave; w++) off += wcnt[w][nn];
-                int ii = off + (int)__popcll(b & lt);
-                double yy[3], cyy[3];
-                bin_point(P, x3, nn, yy, cyy);
-                double fl[3] = {floor(cyy[0]), floor(cyy[1]), floor(cyy[2])};
-                // a replica point passed pointInSupercell, so 0 <= floor < E
-                // and mod(floor + 16E, E) is the floor itself
-                const int cx = P.generic ? cell_of(fl[0], P.Ex) : (int)fl[0];
-                const int cy = P.generic ? cell_of(fl[1], P.Ey) : (int)fl[1];
-                const int cz = P.generic ? cell_of(fl[2], P.Ez) : (int)fl[2];
-                int cell = cx + P.Ex * (cy + P.Ey * cz);
-                pt_cell[ii] = cell;
-                const double f0 = cyy[0] - fl[0], f1 = cyy[1] - fl[1], f2 = cyy[2] - fl[2];
-                pt_frac[3 * (int64_t)ii + 0] = f0;
-                pt_frac[3 * (int64_t)ii + 1] = f1;
-                pt_frac[3 * (int64_t)ii + 2] = f2;
-                const double gd = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
-                if (P.fast) {
-                    // the FAST record: offset from the cell origin in A and
-                    // (cos, sin) of g.r, evaluated in FP64 and rounded once
-                    double sn, cs;
-                    sincos(gd, &sn, &cs);
-                    pt_p32[ii] = make_float4((float)(P.cell_a[0] * f0), (float)(P.cell_a[1] * f1),
-                                             (float)(P.cell_a[2] * f2), (float)cs);
-                    pt_sn32[ii] = (float)sn;
-                } else {
-                    pt_gdotr[ii] = gd;
-                }
-                // arrival slot: lanes of this wave landing in one cell share
-                // one atomic (the lowest such lane adds the group's size);
-                // the grouping is ALU-only, so each lane waits for one atomic
-                // return.  Order inside a cell is restored by ii later.
-                unsigned long long rem = __ballot(1), grp = 0;
-                int leader = lane;
-                while (rem) {
-                    const int l = __ffsll((long long)rem) - 1;
-                    const int lc = __shfl(cell, l, 64);
-                    const unsigned long long gm = __ballot(cell == lc);
-                    if (cell == lc) { leader = l; grp = gm; }
-                    rem &= ~gm;
-                }
-                int base = 0;
-                if (lane == leader) base = atomicAdd(&cell_cnt[cell], (int)__popcll(grp));
-                base = __shfl(base, leader, 64);
-                pt_slot[ii] = base + (int)__popcll(grp & lt);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < P.nuvw) {
-            int s = 0;
-            for (int w = 0; w < BIN_BLOCK / 64; w++) s += wcnt[w][threadIdx.x];
-            run[threadIdx.x] += s;
-        }
-        __syncthreads();
-    }
-}
-
-// Pass 3: place each point at its cell's start + arrival slot.
-__global__ void k_bin_place(int64_t nr, const int* __restrict__ pt_cell, const int* __restrict__ pt_slot,
-                            const int* __restrict__ cell_start, int* __restrict__ cell_ids)
-{
-    int64_t ii = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ii >= nr) return;
-    cell_ids[cell_start[pt_cell[ii]] + pt_slot[ii]] = (int)ii;
-}
-
-// Pass 4a: restore insertion order inside each cell (sort by ii, which is
-// the reference's add order) and the largest cell count (one atomic per
-// wave).  A block takes 256 consecutive cells: their points are one
-// contiguous range, loaded into LDS with coalesced reads when it fits,
-// sorted there segment by segment (one thread per cell: cells of up to
-// SORT_K points are ranked in registers, fully unrolled; larger ones are
-// insertion-sorted), and written back coalesced.  A range that does not fit
-// is sorted in place in memory.
-static constexpr int SORT_K = 16;
-static constexpr int SORT_LDS = 4096;                // ints of LDS per block
-
-// n >= 1.  Every read stays inside the cell: the index is clamped before
-// the select, so a load the compiler issues unconditionally is still in
-// bounds.
-template <typename Ids>
-__device__ __forceinline__ void sort_cell(Ids ids, int s0, int n)
-{
-    if (n <= SORT_K) {
-        int v[SORT_K];
-#pragma unroll
-        for (int k = 0; k < SORT_K; k++) {
-            const int x = ids[s0 + min(k, n - 1)];
-            v[k] = k < n ? x : 0x7fffffff;
-        }
-#pragma unroll
-        for (int i = 0; i < SORT_K; i++) {
-            int r = 0;
-#pragma unroll
-            for (int j = 0; j < SORT_K; j++) r += v[j] < v[i] ? 1 : 0;
-            if (i < n) ids[s0 + r] = v[i];
-        }
-    } else {
-        for (int i = s0 + 1; i < s0 + n; i++) {
-            const int v = ids[i];
-            int k = i - 1;
-            while (k >= s0 && ids[k] > v) { ids[k + 1] = ids[k]; k--; }
-            ids[k + 1] = v;
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_bin_sort(int ncell, const int* __restrict__ cell_start,
-                                                  int* __restrict__ cell_ids, int* __restrict__ blk_max)
-{
-    __shared__ int wmax[4];
-    __shared__ int sid[SORT_LDS];
-    const int c0 = blockIdx.x * blockDim.x;
-    const int c = c0 + threadIdx.x;
-    const int cl = min(c0 + (int)blockDim.x, ncell);
-    const int R0 = cell_start[c0], R1 = cell_start[cl];
-    const bool in_lds = R1 - R0 <= SORT_LDS;
-    int m = 0, s0 = 0, n = 0;
-    if (c < ncell) {
-        s0 = cell_start[c];
-        n = cell_start[c + 1] - s0;
-        m = n;
-    }
-    if (in_lds) {
-        for (int k = R0 + (int)threadIdx.x; k < R1; k += blockDim.x) sid[k - R0] = cell_ids[k];
-        __syncthreads();
-        if (n > 1) sort_cell(sid, s0 - R0, n);
-        __syncthreads();
-        for (int k = R0 + (int)threadIdx.x; k < R1; k += blockDim.x) cell_ids[k] = sid[k - R0];
-    } else if (n > 1) {
-        sort_cell(cell_ids, s0, n);
-    }
-    // the block's largest cell count, stored (no atomics on one address)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) blk_max[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-}
-
-// Largest of n per-block counts into *out (one block).
-__global__ __launch_bounds__(256) void k_max_i32(const int* __restrict__ v, int n, int* __restrict__ out)
-{
-    __shared__ int wmax[4];
-    int m = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) m = max(m, v[i]);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) *out = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-}
-
-// Sum of the FAST guard counters into *out (one block).
-__global__ __launch_bounds__(256) void k_sum_guard(const unsigned long long* __restrict__ part, int n, int stride,
-                                                   unsigned long long* __restrict__ out)
-{
-    __shared__ unsigned long long ws[4];
-    unsigned long long v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[(int64_t)i * stride];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) *out = (ws[0] + ws[1]) + (ws[2] + ws[3]);
-}
-
-// Pass 4b: slot-ordered point data, one thread per slot: 1-based id, FP64
-// offset in cell, FP64 (cos, sin) of g.r, and the FAST record (offset from
-// the cell origin in A for cuboidal cells, cos g.r; sin g.r beside it), all
-// rounded from FP64.
-__global__ __launch_bounds__(256) void k_bin_gather(int64_t nr, const int* __restrict__ cell_ids,
-                                                    const double* __restrict__ pt_frac,
-                                                    const double* __restrict__ pt_gdotr, double a0, double a1,
-                                                    double a2, int* __restrict__ slot_id,
-                                                    double4* __restrict__ slot_frac, double2* __restrict__ slot_cs64,
-                                                    float4* __restrict__ slot_p32, float* __restrict__ slot_sn32)
-{
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nr) return;
-    const int ii = cell_ids[s];
-    slot_id[s] = ii + 1;
-    const double f0 = pt_frac[3 * (int64_t)ii], f1 = pt_frac[3 * (int64_t)ii + 1], f2 = pt_frac[3 * (int64_t)ii + 2];
-    slot_frac[s] = make_double4(f0, f1, f2, 0.0);
-    double sn, cs;
-    sincos(pt_gdotr[ii], &sn, &cs);
-    slot_cs64[s] = make_double2(cs, sn);
-    slot_p32[s] = make_float4((float)(a0 * f0), (float)(a1 * f1), (float)(a2 * f2), (float)cs);
-    slot_sn32[s] = (float)sn;
-}
-
-// Pass 4b, FAST imaging: the point-ordered FAST records copied into slot
-// order (the FP64 offsets stay in point order for the guard re-test).
-__global__ __launch_bounds__(256) void k_bin_gather_fast(int64_t nr, const int* __restrict__ cell_ids,
-                                                         const float4* __restrict__ pt_p32,
-                                                         const float* __restrict__ pt_sn32,
-                                                         float4* __restrict__ slot_p32, float* __restrict__ slot_sn32)
-{
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nr) return;
-    const int ii = cell_ids[s];
-    slot_p32[s] = pt_p32[ii];
-    slot_sn32[s] = pt_sn32[ii];
-}
-
-struct Binned {
-    int64_t nr = 0;
-    int ncell = 0;
-    int max_per_cell = 0;
-};
-
-// Pass 4 (sort within cells, then gather by slot).
-static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s)
-{
-    Workspace& w = g.ws;
-    const int nb = (ncell + 255) / 256;         // blk_max holds >= nb ints (sized with cell_start)
-    hipLaunchKernelGGL(k_bin_sort, dim3(nb), dim3(256), 0, s, ncell, w.cell_start.as<int>(), w.cell_ids.as<int>(),
-                       w.blk_max.as<int>());
-    hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(256), 0, s, w.blk_max.as<int>(), nb, mpc);
-    if (nr > 0 && P.fast)
-        hipLaunchKernelGGL(k_bin_gather_fast, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
-                           w.cell_ids.as<int>(), w.pt_p32.as<float4>(), w.pt_sn32.as<float>(),
-                           w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
-    else if (nr > 0)
-        hipLaunchKernelGGL(k_bin_gather, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
-                           w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(), P.cell_a[0],
-                           P.cell_a[1], P.cell_a[2], w.slot_id.as<int>(), w.slot_frac.as<double4>(),
-                           w.slot_cs64.as<double2>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
-}
-
-// Run binning for either a condition (generic = 0) or plain points.
-static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, hipStream_t s)
-{
-    Workspace& w = g.ws;
-    const int64_t ncell64 = (int64_t)P.Ex * P.Ey * P.Ez;
-    if (ncell64 > (int64_t)INT32_MAX / 2) return seterr(D2BI_ERR_CAPACITY, "too many link cells");
-    const int ncell = (int)ncell64;
-    const int nblk = (int)std::max<int64_t>(1, (n + BIN_TILE - 1) / BIN_TILE);
-    P.n = n;
-    CHK(w.mask.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1)));
-    CHK(w.blk_cnt.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
-    CHK(w.blk_off.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
-    CHK(w.cell_cnt.ensure(sizeof(int) * (ncell + 1)));
-    CHK(w.cell_start.ensure(sizeof(int) * (ncell + 1)));
-    CHK(w.blk_max.ensure(sizeof(int) * ((ncell + 255) / 256 + 1)));
-    CHK(w.counters.ensure(64));
-    HIPCHK(hipMemsetAsync(w.cell_cnt.p, 0, sizeof(int) * (ncell + 1), s));
-    HIPCHK(hipMemsetAsync(w.counters.p, 0, 64, s));
-    if (n > 0) {
-        hipLaunchKernelGGL(k_bin_classify, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
-                           w.mask.as<uint32_t>(), w.blk_cnt.as<int>(), nblk);
-        HIPCHK(hipGetLastError());
-    } else {
-        HIPCHK(hipMemsetAsync(w.blk_cnt.p, 0, sizeof(int) * ((int64_t)P.nuvw * nblk + 1), s));
-    }
-    const int64_t nbc = (int64_t)P.nuvw * nblk;
-    CHK(scan_exclusive(w.blk_cnt.as<int>(), w.blk_off.as<int>(), nbc, s));
-    int nr_host = 0;
-    HIPCHK(hipMemcpyAsync(&nr_host, w.blk_off.as<int>() + nbc, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (nr_host < 0) return seterr(D2BI_ERR_CAPACITY, "point count overflow");
-    const int64_t nr = nr_host;
-    const int64_t nrc = std::max<int64_t>(nr, 1);
-    CHK(w.pt_cell.ensure(sizeof(int) * nrc));
-    CHK(w.pt_slot.ensure(sizeof(int) * nrc));
-    CHK(w.pt_frac.ensure(sizeof(double) * 3 * nrc));
-    CHK(w.cell_ids.ensure(sizeof(int) * nrc));
-    CHK(w.slot_p32.ensure(sizeof(float4) * nrc));
-    CHK(w.slot_sn32.ensure(sizeof(float) * nrc));
-    if (P.fast) {
-        CHK(w.pt_p32.ensure(sizeof(float4) * nrc));
-        CHK(w.pt_sn32.ensure(sizeof(float) * nrc));
-    } else {
-        CHK(w.pt_gdotr.ensure(sizeof(double) * nrc));
-        CHK(w.slot_id.ensure(sizeof(int) * nrc));
-        CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
-        CHK(w.slot_cs64.ensure(sizeof(double2) * nrc));
-    }
-    if (n > 0) {
-        hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
-                           w.mask.as<uint32_t>(), w.blk_off.as<int>(), nblk, w.pt_cell.as<int>(),
-                           w.pt_slot.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(),
-                           w.pt_p32.as<float4>(), w.pt_sn32.as<float>(), w.cell_cnt.as<int>());
-        HIPCHK(hipGetLastError());
-    }
-    CHK(scan_exclusive(w.cell_cnt.as<int>(), w.cell_start.as<int>(), ncell, s));
-    if (nr > 0) {
-        hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
-                           w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.cell_start.as<int>(),
-                           w.cell_ids.as<int>());
-        HIPCHK(hipGetLastError());
-    }
-    launch_bin_finish(ncell, nr, P, w.counters.as<int>(), s);
-    HIPCHK(hipGetLastError());
-    out->nr = nr;
-    out->ncell = ncell;
-    return D2BI_OK;
-}
-
-// Fill the binning parameters of a condition from the plan.
-static int bin_params(const d2bi_record* r, const d2bi_plan* p, BinParams* P)
-{
-    std::memset(P, 0, sizeof *P);
-    if (r->nuvw < 1 || r->nuvw > MAX_REPL)
-        return seterr(D2BI_ERR_ARG, "nuvw = %d not supported (1..%d)", r->nuvw, MAX_REPL);
-    std::memcpy(P->RU, p->RU, sizeof P->RU);
-    std::memcpy(P->delta, r->delta, sizeof P->delta);
-    std::memcpy(P->lc_delta, p->lc_delta, sizeof P->lc_delta);
-    std::memcpy(P->ia, p->ia_col, sizeof P->ia);
-    std::memcpy(P->gg, p->gg, sizeof P->gg);
-    for (int nn = 0; nn < r->nuvw; nn++) {
-        const int* u = &r->uvw[3 * nn];
-        for (int i = 0; i < 3; i++) {       // getSuperVector (Lib_SimpleSupercells.f90:904-918)
-            double s1 = r->super_a[i] * r->super_n[0];
-            double s2 = r->super_a[3 + i] * r->super_n[1];
-            double s3 = r->super_a[6 + i] * r->super_n[2];
-            P->Ruvw[3 * nn + i] = (u[0] * s1 + u[1] * s2) + u[2] * s3;
-        }
-    }
-    P->nuvw = r->nuvw;
-    P->Ex = p->Ex; P->Ey = p->Ey; P->Ez = p->Ez;
-    P->generic = 0;
-    P->cell_a[0] = p->a_col[0]; P->cell_a[1] = p->a_col[4]; P->cell_a[2] = p->a_col[8];
-    // folded transform of the pre-classification (any rounding order: it
-    // only decides replicas that are further than eps from a box face)
-    double ia_max = 0.0, ru_max = 0.0, t_max = 0.0;
-    for (int i = 0; i < 9; i++) {
-        ia_max = std::max(ia_max, std::fabs(P->ia[i]));
-        ru_max = std::max(ru_max, std::fabs(P->RU[i]));
-    }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double v = 0.0;
-            for (int k = 0; k < 3; k++) v += P->ia[3 * k + i] * P->RU[3 * j + k];
-            P->M[3 * j + i] = v;
-        }
-    for (int nn = 0; nn < P->nuvw; nn++) {
-        double z[3];
-        for (int k = 0; k < 3; k++) {
-            double v = P->delta[k] - P->lc_delta[k];
-            double t = std::fabs(P->delta[k]) + std::fabs(P->lc_delta[k]);
-            for (int j = 0; j < 3; j++) {
-                v += P->RU[3 * j + k] * P->Ruvw[3 * nn + j];
-                t += std::fabs(P->RU[3 * j + k] * P->Ruvw[3 * nn + j]) + std::fabs(P->Ruvw[3 * nn + j]);
-            }
-            z[k] = v;
-            t_max = std::max(t_max, t);
-        }
-        for (int i = 0; i < 3; i++) {
-            double v = 0.0;
-            for (int k = 0; k < 3; k++) v += P->ia[3 * k + i] * z[k];
-            P->off[3 * nn + i] = v;
-        }
-    }
-    // bounds on the magnitude of every intermediate of both evaluations:
-    // |x| terms scale with 3 max|ia| max(1, max|RU|), the rest with
-    // 3 max|ia| (|RU Ruvw| + |Ruvw| + |delta| + |lc_delta|)
-    P->mabs = 3.0 * ia_max * std::max(1.0, ru_max) * std::max(1.0, 3.0 * ru_max);
-    P->oabs = 3.0 * ia_max * t_max;
-    // box faces at 0 and E <= 2^20, so that E + eps and the exact path's
-    // integer test are exact
-    P->prefilter = (P->Ex <= (1 << 20) && P->Ey <= (1 << 20) && P->Ez <= (1 << 20)) ? 1 : 0;
-    return D2BI_OK;
-}
-
-// ------------------------------------------------------------------ voxel axis tables
-
-// The voxel and link cells are cuboidal (findTransformedSupercell gives a
-// diagonal D, Lib_DiffractionConditions.f90:1074-1077), so a voxel's link
-// cell and in-cell offset factorise per axis: for axis x,
-//   xx = a_vox(1,1)*ix - lc_delta(1),  zi = ia(1,1)*xx,
-//   cell = mod(floor(zi) + 16*Ex, Ex), zi = zi - cell
-// (neighbourList1b, Lib_LinkCell3D.f90:1549-1565).  The zero off-diagonal
-// terms of the reference's 3x3 products add +-0 and change no bit.
-static constexpr int TILE = 4;                       // FAST voxel tiles: 4 voxels along x and y (z: 4 or 8)
-
-// FP32 voxel position along one axis relative to the link cell of its
-// FAST tile's first voxel, and that cell offset
-struct QL {
-    float q;
-    int lc;
-};
-
-struct AxisTables {
-    std::vector<int> cell[3];
-    std::vector<double> zi[3];
-    std::vector<QL> ql[4];         // x, y, z for 4-voxel tiles; z for 8-voxel tiles
-    std::vector<int> start[3];     // first voxel of each link cell (size E+1)
-    int lo[3], hi[3];              // active link-cell range
-};
-
-static bool diagonal(const double* m)
-{
-    return m[1] == 0.0 && m[2] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[6] == 0.0 && m[7] == 0.0;
-}
-
-static int axis_tables(const d2bi_record* r, const d2bi_plan* p, AxisTables* t)
-{
-    if (!diagonal(r->gk_a) || !diagonal(p->a_col) || !diagonal(p->ia_col))
-        return seterr(D2BI_ERR_ARG, "voxel/link cells must be cuboidal (diagonal cell matrices)");
-    const int M[3] = {p->Mx, p->My, p->Mz};
-    const int E[3] = {p->Ex, p->Ey, p->Ez};
-    for (int a = 0; a < 3; a++) {
-        t->cell[a].resize(M[a]);
-        t->zi[a].resize(M[a]);
-        t->start[a].assign(E[a] + 1, M[a]);
-        const double av = r->gk_a[4 * a], ia = p->ia_col[4 * a];
-        for (int i = 0; i < M[a]; i++) {
-            double xx = av * i - p->lc_delta[a];
-            double zi = ia * xx;
-            int c = ((int)floor(zi) + 16 * E[a]) % E[a];
-            t->cell[a][i] = c;
-            t->zi[a][i] = zi - c;
-        }
-        for (int i = 1; i < M[a]; i++)
-            if (t->cell[a][i] < t->cell[a][i - 1])
-                return seterr(D2BI_ERR_ARG, "voxel column wraps the link-cell box (axis %d)", a);
-        t->ql[a].resize(M[a]);
-        const double side = p->a_col[4 * a];
-        for (int i = 0; i < M[a]; i++) {
-            const int lc = t->cell[a][i] - t->cell[a][(i / TILE) * TILE];
-            t->ql[a][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
-        }
-        if (a == 2) {
-            // z again, relative to the cell of 8-voxel tile starts (4 x 4 x 8 tiles)
-            t->ql[3].resize(M[a]);
-            for (int i = 0; i < M[a]; i++) {
-                const int lc = t->cell[a][i] - t->cell[a][(i / 8) * 8];
-                t->ql[3][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
-            }
-        }
-        for (int i = M[a] - 1; i >= 0; i--) t->start[a][t->cell[a][i]] = i;
-        // cells with no voxels take the start of the next occupied cell
-        for (int c = E[a] - 1; c >= 0; c--)
-            if (t->start[a][c] > t->start[a][c + 1]) t->start[a][c] = t->start[a][c + 1];
-        t->lo[a] = t->cell[a][0];
-        t->hi[a] = t->cell[a][M[a] - 1];
-    }
-    return D2BI_OK;
-}
-
-// Upload tables: ints [cellx|celly|cellz], doubles [zix|ziy|ziz], ints [startx|starty|startz]
-struct AxisDev {
-    const int* cell[3];
-    const double* zi[3];
-    const int* start[3];
-    const QL* ql[4];
-};
-
-static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
-{
-    Workspace& w = g.ws;
-    size_t nc = t.cell[0].size() + t.cell[1].size() + t.cell[2].size();
-    size_t ns = t.start[0].size() + t.start[1].size() + t.start[2].size();
-    size_t nq = nc + t.ql[3].size();
-    CHK(w.ax_cell.ensure(sizeof(int) * nc));
-    CHK(w.ax_zi.ensure(sizeof(double) * nc));
-    CHK(w.ax_start.ensure(sizeof(int) * ns));
-    CHK(w.ax_ql.ensure(sizeof(QL) * nq));
-    size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns + sizeof(QL) * nq;
-    void* hp;
-    CHK(pinned(bytes, &hp));
-    char* h = (char*)hp;
-    int* hc = (int*)h;
-    double* hz = (double*)(h + sizeof(int) * nc);
-    int* hs = (int*)(h + sizeof(int) * nc + sizeof(double) * nc);
-    QL* hq = (QL*)(h + sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns);
-    size_t oc = 0, os = 0;
-    for (int a = 0; a < 3; a++) {
-        std::memcpy(hc + oc, t.cell[a].data(), sizeof(int) * t.cell[a].size());
-        std::memcpy(hz + oc, t.zi[a].data(), sizeof(double) * t.zi[a].size());
-        std::memcpy(hs + os, t.start[a].data(), sizeof(int) * t.start[a].size());
-        std::memcpy(hq + oc, t.ql[a].data(), sizeof(QL) * t.ql[a].size());
-        d->ql[a] = w.ax_ql.as<QL>() + oc;
-        if (a == 2) {
-            std::memcpy(hq + nc, t.ql[3].data(), sizeof(QL) * t.ql[3].size());
-            d->ql[3] = w.ax_ql.as<QL>() + nc;
-        }
-        d->cell[a] = w.ax_cell.as<int>() + oc;
-        d->zi[a] = w.ax_zi.as<double>() + oc;
-        d->start[a] = w.ax_start.as<int>() + os;
-        oc += t.cell[a].size();
-        os += t.start[a].size();
-    }
-    HIPCHK(hipMemcpyAsync(w.ax_cell.p, hc, sizeof(int) * nc, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(w.ax_zi.p, hz, sizeof(double) * nc, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(w.ax_start.p, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(w.ax_ql.p, hq, sizeof(QL) * nq, hipMemcpyHostToDevice, s));
-    // the pinned buffer is reused by later calls: they wait for this event
-    HIPCHK(hipEventRecord(g.ev[9], s));
-    g.pinned_busy = true;
-    return D2BI_OK;
-}
-
-struct VoxParams {
-    int Mx, My, Mz;
-    int Ex, Ey, Ez;
-    double a[3];                // link-cell sides (diagonal of a_col)
-    double rc2, i2s2, sigma;
-    int use_density;
-    AxisDev ax;
-};
-
-// convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994)
-__device__ inline double rho_of(double r2_min, double sig)
-{
-    if (r2_min < sig * sig) return 1.0;
-    if (r2_min > 4 * sig * sig) return 0.0;
-    double xx = sqrt(r2_min) / sig - 1.0;
-    return 1 - ((xx * xx) * xx) * ((10 - 15 * xx) + (6 * xx) * xx);
-}
-
-// ------------------------------------------------------------------ (b) voxelise, FP64
-
-// One thread per voxel; the 27 neighbour cells in (kz,ky,kx) order and the
-// points of each cell in insertion order, exactly the neighbour list order
-// of neighbourList1b, summed as the reference sums it (:1877-1892).
-__global__ __launch_bounds__(256) void k_voxel_fp64(VoxParams P, const int* __restrict__ cell_start,
-                                                    const double4* __restrict__ frac,
-                                                    const double2* __restrict__ cs64,
-                                                    double* __restrict__ out_re,
-                                                    double* __restrict__ out_im,
-                                                    double* __restrict__ out_rho,
-                                                    int* __restrict__ out_cnt,
-                                                    double* __restrict__ out_r2m)
-{
-    const int ix = blockIdx.x * 8 + (threadIdx.x & 7);
-    const int iy = blockIdx.y * 8 + ((threadIdx.x >> 3) & 7);
-    const int iz = blockIdx.z * 4 + (threadIdx.x >> 6);
-    if (ix >= P.Mx || iy >= P.My || iz >= P.Mz) return;
-    const int cx = P.ax.cell[0][ix], cy = P.ax.cell[1][iy], cz = P.ax.cell[2][iz];
-    const double zx = P.ax.zi[0][ix], zy = P.ax.zi[1][iy], zz = P.ax.zi[2][iz];
-    double sre = 0.0, sim = 0.0, r2min = 3.4028234663852886e38;
-    int n = 0;
-    for (int kz = -1; kz <= 1; kz++) {
-        const int jz = (cz + P.Ez + kz) % P.Ez;
-        const double dzi3 = kz - zz;
-        for (int ky = -1; ky <= 1; ky++) {
-            const int jy = (cy + P.Ey + ky) % P.Ey;
-            const double dzi2 = ky - zy;
-            for (int kx = -1; kx <= 1; kx++) {
-                const int jx = (cx + P.Ex + kx) % P.Ex;
-                const double dzi1 = kx - zx;
-                const int c = jx + P.Ex * (jy + P.Ey * jz);
-                const int s1 = cell_start[c + 1];
-                for (int s = cell_start[c]; s < s1; s++) {
-                    const double4 f = frac[s];
-                    const double d1 = P.a[0] * (f.x + dzi1);
-                    const double d2 = P.a[1] * (f.y + dzi2);
-                    const double d3 = P.a[2] * (f.z + dzi3);
-                    const double dd = (d1 * d1 + d2 * d2) + d3 * d3;
-                    if (dd <= P.rc2) {
-                        n++;
-                        r2min = r2min < dd ? r2min : dd;
-                        const double e = exp(-dd * P.i2s2);
-                        const double2 cs = cs64[s];
-                        sre = sre + e * cs.x;
-                        sim = sim + e * cs.y;
-                    }
-                }
-            }
-        }
-    }
-    const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
-    double ww = hypot(sre, sim);
-    ww = 1 / (1.0e-16 > ww ? 1.0e-16 : ww);
-    out_re[o] = sre * ww;
-    out_im[o] = sim * ww;
-    out_rho[o] = P.use_density ? rho_of(r2min, P.sigma) : 1.0;
-    if (out_cnt) out_cnt[o] = n;
-    if (out_r2m) out_r2m[o] = r2min;
-}
-
-// ------------------------------------------------------------------ (c) integrate, FP64
-
-struct IntParams {
-    int Mx, My, Mz, nPrec;
-    double b0, bg, dz;
-    const double* aa;           // device array of 2 pi s_g per precession point
-    int accumulate;             // add the column sum to img instead of writing
-    double div;                 // written value: sum / div
-};
-
-// dynamicalTwoBeamIntegrationColumn_phaserho (Lib_DynamicalTwoBeamImaging.f90:2316-2480):
-// one thread per pixel column, per-double-step renormalisation, precession
-// average (:1897-1900, 1951).
-__global__ __launch_bounds__(256) void k_integrate_fp64(IntParams P, const double* __restrict__ fre,
-                                                        const double* __restrict__ fim,
-                                                        const double* __restrict__ frho,
-                                                        double* __restrict__ img)
-{
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t plane = (int64_t)P.Mx * P.My;
-    if (col >= plane) return;
-    const int nz = P.Mz;
-    const double b0 = P.b0, bg = P.bg, dz = P.dz;
-    double acc = 0.0;
-    for (int prec = 0; prec < P.nPrec; prec++) {
-        const double a = P.aa[prec];
-        double phi0r = 1.0, phi0i = 0.0, phigr = 0.0, phigi = 0.0;
-        double dphi0r, dphi0i, dphigr, dphigi;
-        double phi0r1, phi0i1, phigr1, phigi1, phi0r2, phi0i2, phigr2, phigi2;
-        // coefficients at sample k (sample Mz is sample 0, :1894-1895)
-        auto coef = [&](int k, double& c0, double& ca, double& cgr, double& cgi) {
-            const int64_t o = col + plane * (k == nz ? 0 : k);
-            const double rh = frho[o];
-            c0 = b0 * rh;
-            ca = b0 * rh + a;
-            cgr = (bg * rh) * fre[o];
-            cgi = (bg * rh) * fim[o];
-        };
-        double c0a, caa, cgra, cgia;
-        if (nz >= 2) coef(0, c0a, caa, cgra, cgia);
-        for (int ii = 0; ii <= nz - 2; ii += 2) {
-            double c0b, cab, cgrb, cgib, c0c, cac, cgrc, cgic;
-            coef(ii + 1, c0b, cab, cgrb, cgib);
-            coef(ii + 2, c0c, cac, cgrc, cgic);
-#define DERIV(c0, ca, cgr, cgi)                                      \
-    dphi0r = ((-(c0 * phi0i)) - cgi * phigr) - cgr * phigi;          \
-    dphi0i = ((c0 * phi0r) + cgr * phigr) - cgi * phigi;             \
-    dphigr = ((cgi * phi0r) - cgr * phi0i) - ca * phigi;             \
-    dphigi = ((cgr * phi0r) + cgi * phi0i) + ca * phigr;
-            DERIV(c0a, caa, cgra, cgia)
-            phi0r2 = phi0r; phi0i2 = phi0i; phigr2 = phigr; phigi2 = phigi;
-            phi0r1 = phi0r + (dphi0r * dz) / 3; phi0i1 = phi0i + (dphi0i * dz) / 3;
-            phigr1 = phigr + (dphigr * dz) / 3; phigi1 = phigi + (dphigi * dz) / 3;
-            phi0r = phi0r + dphi0r * dz; phi0i = phi0i + dphi0i * dz;
-            phigr = phigr + dphigr * dz; phigi = phigi + dphigi * dz;
-            DERIV(c0b, cab, cgrb, cgib)
-            phi0r = phi0r2 + dphi0r * dz; phi0i = phi0i2 + dphi0i * dz;
-            phigr = phigr2 + dphigr * dz; phigi = phigi2 + dphigi * dz;
-            phi0r1 = phi0r1 + ((dphi0r * 2) * dz) / 3; phi0i1 = phi0i1 + ((dphi0i * 2) * dz) / 3;
-            phigr1 = phigr1 + ((dphigr * 2) * dz) / 3; phigi1 = phigi1 + ((dphigi * 2) * dz) / 3;
-            DERIV(c0b, cab, cgrb, cgib)
-            phi0r = phi0r2 + (dphi0r * 2) * dz; phi0i = phi0i2 + (dphi0i * 2) * dz;
-            phigr = phigr2 + (dphigr * 2) * dz; phigi = phigi2 + (dphigi * 2) * dz;
-            phi0r1 = phi0r1 + ((dphi0r * 2) * dz) / 3; phi0i1 = phi0i1 + ((dphi0i * 2) * dz) / 3;
-            phigr1 = phigr1 + ((dphigr * 2) * dz) / 3; phigi1 = phigi1 + ((dphigi * 2) * dz) / 3;
-            DERIV(c0c, cac, cgrc, cgic)
-#undef DERIV
-            phi0r = phi0r1 + (dphi0r * dz) / 3; phi0i = phi0i1 + (dphi0i * dz) / 3;
-            phigr = phigr1 + (dphigr * dz) / 3; phigi = phigi1 + (dphigi * dz) / 3;
-            double phimag = ((phi0r * phi0r + phi0i * phi0i) + phigr * phigr) + phigi * phigi;
-            phimag = 1 / sqrt(phimag);
-            phi0r = phi0r * phimag; phi0i = phi0i * phimag;
-            phigr = phigr * phimag; phigi = phigi * phimag;
-            c0a = c0c; caa = cac; cgra = cgrc; cgia = cgic;
-        }
-        acc = acc + (phigr * phigr + phigi * phigi);
-    }
-    img[col] = P.accumulate ? img[col] + acc : acc / P.div;
-}
-
-// ------------------------------------------------------------------ (b) voxelise, FAST
-
-static constexpr int TILE_CAP = 160;                 // staged candidates per pass (lane per voxel)
-static constexpr int GUARD_SLOTS = 1024;             // guard counters (power of two) ...
-static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart
-static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
-
-// Inclusive prefix sum over the 64 lanes of a wave in six DPP steps: within
-// each row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill), then row 0's
-// total into row 1 and row 2's into row 3 (row_bcast:15), then rows 0-1's
-// into rows 2-3 (row_bcast:31).  All lanes must be active.
-__device__ __forceinline__ int wave_incl_scan(int v)
-{
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
-    return v;
-}
-
-struct FastParams {
-    VoxParams v;
-    int ntx, nty, ntz;          // tiles per axis
-    int ntiles, per_xcd;
-    float e_rc, e_band;         // k2 (3 sigma)^2 and the guard half-width around it
-    float rc_cull2;             // (3 sigma)^2 widened for the tile-box cull
-    float k2;                   // -log2(e)/(2 sigma^2)
-    float w_scale;              // 2^e_rc
-    float m2_min;               // (1e-16)^2 / w_scale^2
-    float inv_k2;               // 1 / k2
-    float cellf[3];             // link-cell sides in FP32
-    int thin_z;                 // link cells thinner than 3 sigma along z
-    int tz;                     // tile depth in voxels (4, or 8 for the 4 x 4 x 8 row kernel)
-};
-
-// One wave per voxel tile: 4x4x4 voxels, or 4x4x8 for the row kernel with
-// TZ = 8 (x, y, z; x fastest).  The tile's candidates are the points of
-// every link cell in the unwrapped range [cell(first voxel)-1,
-// cell(last voxel)+1] on each axis; an unwrapped cell u stands for cell
-// mod(u, E) displaced by u - mod(u, E) cells, which is exactly the periodic
-// image neighbourList1b sees through its mod() (Lib_LinkCell3D.f90:1567-1575).
-// Cells, then points, farther than 3 sigma from the tile's voxel box are
-// dropped; the rest are compacted into LDS (ballot + popcount) and every
-// voxel of the tile tests them all.
-//
-// Staging.  The kept non-empty cells are listed once per tile with their
-// origin in the tile frame and the offset from candidate index to point
-// slot.  Candidates are then taken 64 at a time; the cell of candidate t is
-// the number of kept cells starting at or before t, less one, found from a
-// 64-bit mask of the cells starting inside the round (a flag per lane, a
-// ballot and a popcount).
-//
-// Arithmetic.  Positions are taken relative to the tile centre c (FP32,
-// rounded once from FP64 values).  For a candidate p and voxel q the Gaussian
-// exponent e = k2*|p - q|^2 (k2 = -log2(e)/(2 sigma^2), so the weight is 2^e)
-// is evaluated as
-//     e = (-2 k2 p).q + (k2 |p|^2 + k2 |q|^2)
-// from a per-candidate record (-2 k2 p, k2 |p|^2): three FMAs and an add.
-// Membership is e >= k2 (3 sigma)^2; where |e - k2 (3 sigma)^2| is within
-// the guard band the pair is re-tested in FP64 with the expression of
-// neighbourList1b (and the neighbour offset must be -1, 0 or +1, as in its
-// 27-cell loop), so the set of summed atoms is exactly the FP64 set.
-// cos/sin of g.r were evaluated in FP64.
-//
-// ROW = false: lane = voxel; S = sum w (cos, sin) is accumulated in blocks of
-// 32 candidates whose sums are merged with Kahan compensation.
-//
-// ROW = true: the wave is the tile's voxel rows along x (one per (y, z): 16
-// for TZ = 4, 32 for TZ = 8) times NQ candidate parts (4 or 2); lane (row, s)
-// walks candidates s, s+NQ, s+2NQ, ... for the four voxels of its row.  The
-// Gaussian factorises along x:
-//     2^(e_i - e_rc) = X_i * 2^t,   X_i = 2^(k2 (p_x - q_x,i)^2),
-//     t = k2 ((p_y - q_y)^2 + (p_z - q_z)^2) - e_rc   (e_rc = k2 (3 sigma)^2),
-// and the four x positions q_x,i are the tile's, common to every row, so
-// X_0..X_3 are evaluated once per candidate when it is staged and a lane
-// spends one v_exp_f32 per candidate on four voxels.  Membership e_i >= e_rc
-// is u_i = X_i 2^t - 1 >= 0; a block in which some |u_i| lies within the
-// guard band is walked again and the pairs of the band are re-tested in FP64,
-// correcting the sums where the FP64 test disagrees.  The parts' sums are
-// added in a fixed order through LDS at the end, after which lane (row, s)
-// holds voxels s NV .. s NV + NV - 1 of its row (NV = 4 / NQ).
-//
-// Thin cells.  Link cells are at least 3 sigma wide along x and y (Ex <=
-// Dx/deltaM + 2 with deltaM >= 3 sigma / min(a_x, a_y)), so along those axes
-// every point within 3 sigma of a voxel lies in a neighbouring cell and the
-// range test alone decides membership (the guard band covers rounding of the
-// cell sides).  Along z the cell height Mz a_z / Ez can be smaller (a_z <
-// min(a_x, a_y) with deltaM dividing Mz, or Ez = 3 for thin foils); then a
-// point two cells away can lie within 3 sigma, and neighbourList1b does not
-// see it.  ROW = 2 applies that rule: a candidate whose cell is not within one
-// cell of the row's along z gets weight 0 for the row.  The diagnostic kernel
-// tests the z offset of every pair when F.thin_z is set.
-template <bool DENSITY, bool DIAG, int ROW, int TZ>
-__global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
-                                                   const int* __restrict__ cell_ids,
-                                                   const double* __restrict__ pt_frac,
-                                                   const float4* __restrict__ p32,
-                                                   const float* __restrict__ sn32,
-                                                   float4* __restrict__ field,
-                                                   int* __restrict__ out_cnt,
-                                                   float* __restrict__ out_r2m,
-                                                   unsigned long long* __restrict__ guard_count)
-{
-    static_assert(!(DIAG && ROW), "the diagnostic kernel tests every pair itself");
-    static_assert(TZ == 4 || (ROW && TZ == 8), "tiles of 4 x 4 x 4, or 4 x 4 x 8 for the row kernel");
-    // ROW: candidate parts per row (4 for 16 rows, 2 for 32 rows), voxels a
-    // lane finishes (4 / NQ)
-    constexpr int NQ = ROW ? (TZ == 8 ? 2 : 4) : 1;
-    constexpr int NV = ROW ? 4 / NQ : 1;
-    // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
-    // give each XCD a contiguous range of tiles so neighbouring tiles, which
-    // read the same link cells, share one L2.
-    const int bid = blockIdx.x;
-    const int L = (bid & 7) * F.per_xcd + (bid >> 3);
-    if (L >= F.ntiles) return;
-    const VoxParams& P = F.v;
-    const int tx = L % F.ntx, ty = (L / F.ntx) % F.nty, tz = L / (F.ntx * F.nty);
-    const int lane = threadIdx.x;
-    const int x0 = tx * TILE, y0 = ty * TILE, z0 = tz * TZ;
-    const int x1 = min(P.Mx, x0 + TILE) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TZ) - 1;
-    // the lane's row of voxels along x (y fastest) and, lane per voxel, its x
-    const int rrow = ROW ? lane / NQ : (lane >> 2);
-    const int ix = x0 + (lane & 3), iy = y0 + (rrow & 3), iz = z0 + (rrow >> 2);
-    const bool row_act = iy <= y1 && iz <= z1;
-    const bool act = ix <= x1 && row_act;
-    const int jx = min(ix, x1), jy = min(iy, y1), jz = min(iz, z1);
-    const QL* const qlz_tab = TZ == 8 ? P.ax.ql[3] : P.ax.ql[2];   // z relative to the tile's first cell
-
-    // origin cell of the tile; the voxel's FP32 position relative to it and
-    // its cell offset (host tables)
-    const int c0x = P.ax.cell[0][x0], c0y = P.ax.cell[1][y0], c0z = P.ax.cell[2][z0];
-    const QL qlx = P.ax.ql[0][jx], qly = P.ax.ql[1][jy], qlz = qlz_tab[jz];
-    const int lcx = qlx.lc, lcy = qly.lc, lcz = qlz.lc;
-    const float qx = qlx.q, qy = qly.q, qz = qlz.q;
-    // the tile's voxel box: positions do not decrease with the voxel index
-    // (link cells are monotonic along each axis), so its corners are the
-    // first and last voxels
-    const QL ex0 = P.ax.ql[0][x0], ex1 = P.ax.ql[0][x1];
-    const QL ey0 = P.ax.ql[1][y0], ey1 = P.ax.ql[1][y1];
-    const QL ez0 = qlz_tab[z0], ez1 = qlz_tab[z1];
-    const float bx0 = ex0.q, bx1 = ex1.q, by0 = ey0.q, by1 = ey1.q, bz0 = ez0.q, bz1 = ez1.q;
-    // tile-centred frame
-    const float ccx = 0.5f * (bx0 + bx1), ccy = 0.5f * (by0 + by1), ccz = 0.5f * (bz0 + bz1);
-    const float hx = bx1 - ccx, hy = by1 - ccy, hz = bz1 - ccz;
-    const float qcx = qx - ccx, qcy = qy - ccy, qcz = qz - ccz;
-    const float k2 = F.k2;
-    const float kq2 = k2 * fmaf(qcx, qcx, fmaf(qcy, qcy, qcz * qcz));
-    // ROW: the tile's four x positions (clamped at the grid's edge), the
-    // lane's candidate part and the y-z part of its exponent
-    float qxt[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) qxt[i] = P.ax.ql[0][min(x0 + i, x1)].q - ccx;
-    const int squart = lane & (NQ - 1);
-    const float kyz = fmaf(k2, fmaf(qcy, qcy, qcz * qcz), -F.e_rc);
-
-    // unwrapped cell range (cells are monotonic, so ux0 + dx lies in [-1, E])
-    const int ux0 = c0x - 1, uy0 = c0y - 1, uz0 = c0z - 1;
-    const int ncx = ex1.lc + 3, ncy = ey1.lc + 3, ncz = ez1.lc + 3;
-    const int ncell = ncx * ncy * ncz;                 // <= TILE_MAXC (checked on the host)
-
-    // staged candidates: ROW = false, (-2 k2 p, k2 |p|^2) tile-centred, then
-    // (cos, sin) of g.r, slot, cell offsets; ROW = true, (X_0..X_3), then
-    // (-2 k2 p_y, -2 k2 p_z, k2 (p_y^2 + p_z^2), cos), then (sin, slot, cell
-    // offsets, 0), and four zero records after the last candidate of a pass
-    constexpr int REC = ROW ? 3 : 2;
-    // ROW: 96 staged candidates per pass keep the wave's LDS at 7.6 KB, so
-    // its 88 VGPRs, not LDS, bound occupancy (20 waves per CU; 128 allowed 17
-    // and ran 15 % slower on c1)
-    constexpr int CAP = ROW ? 96 : TILE_CAP;
-    constexpr int NREC = ROW ? CAP + 4 : CAP;
-    __shared__ float4 crec[TILE_MAXC];         // kept cells: origin in the tile frame, slot - index
-    __shared__ int cupk[TILE_MAXC];            // kept cells: dx | dy << 8 | dz << 16
-    __shared__ int sflag[64];                  // cells starting at a lane's candidate this round
-    __shared__ float4 sA[REC * NREC];
-    int* const cstart = reinterpret_cast<int*>(sA);    // kept cells' first candidate (before staging)
-    static_assert(sizeof(int) * TILE_MAXC <= sizeof(float4) * REC * NREC, "cstart overlay");
-    static_assert(!ROW || 3 * 256 * sizeof(float) <= sizeof(float4) * REC * NREC, "row sums overlay");
-
-    // set bits of a wave mask below this lane (v_mbcnt)
-    auto below = [](unsigned long long m) -> int {
-        return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-    };
-
-    // kept cells: in range of the tile box and not empty, in cell order
-    int K = 0, T = 0;
-    {
-        // (c + 0.5) / n lies at least 1/(2n) >= 1/256 from an integer, far
-        // beyond the error of v_rcp_f32, so the truncations below are exact
-        const float inx = __builtin_amdgcn_rcpf((float)ncx), inxy = __builtin_amdgcn_rcpf((float)(ncx * ncy));
-        for (int cb = 0; cb < ncell; cb += 64) {
-            const int c = cb + lane;
-            int cnt = 0, b = 0, upk = 0;
-            float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-            if (c < ncell) {
-                // c = dx + ncx (dy + ncy dz); c < 128 so the float quotients are exact
-                const int dz = (int)(((float)c + 0.5f) * inxy);
-                const int r = c - dz * (ncx * ncy);
-                const int dy = (int)(((float)r + 0.5f) * inx);
-                const int dx = r - dy * ncx;
-                lx = F.cellf[0] * (float)(dx - 1) - ccx;
-                ly = F.cellf[1] * (float)(dy - 1) - ccy;
-                lz = F.cellf[2] * (float)(dz - 1) - ccz;
-                const float gx = fmaxf(0.0f, fmaxf(lx - hx, -hx - (lx + F.cellf[0])));
-                const float gy = fmaxf(0.0f, fmaxf(ly - hy, -hy - (ly + F.cellf[1])));
-                const float gz = fmaxf(0.0f, fmaxf(lz - hz, -hz - (lz + F.cellf[2])));
-                if (fmaf(gx, gx, fmaf(gy, gy, gz * gz)) <= F.rc_cull2) {
-                    int rx = ux0 + dx, ry = uy0 + dy, rz = uz0 + dz;
-                    rx = rx < 0 ? rx + P.Ex : (rx >= P.Ex ? rx - P.Ex : rx);
-                    ry = ry < 0 ? ry + P.Ey : (ry >= P.Ey ? ry - P.Ey : ry);
-                    rz = rz < 0 ? rz + P.Ez : (rz >= P.Ez ? rz - P.Ez : rz);
-                    const int rc = rx + P.Ex * (ry + P.Ey * rz);
-                    b = cell_start[rc];
-                    cnt = cell_start[rc + 1] - b;
-                    upk = dx | (dy << 8) | (dz << 16);
-                }
-            }
-            const int incl = wave_incl_scan(cnt);
-            const unsigned long long m = __ballot(cnt > 0);
-            if (cnt > 0) {
-                const int k = K + below(m);
-                const int st = T + incl - cnt;
-                cstart[k] = st;
-                crec[k] = make_float4(lx, ly, lz, __int_as_float(b - st));
-                cupk[k] = upk;
-            }
-            K += (int)__popcll(m);
-            T += __shfl(incl, 63, 64);
-        }
-    }
-    __syncthreads();
-    T = __builtin_amdgcn_readfirstlane(T);
-    K = __builtin_amdgcn_readfirstlane(K);
-    // starts of kept cells lane and lane + 64 (K <= 128)
-    const int s_lo = lane < K ? cstart[lane] : 0x7fffffff;
-    const int s_hi = lane + 64 < K ? cstart[lane + 64] : 0x7fffffff;
-    __syncthreads();
-
-    // exact FP64 test of neighbourList1b (Lib_LinkCell3D.f90:1565-1596) for
-    // a staged candidate (packed unwrapped cell offsets u, point slot) and
-    // the voxel of x index vx (cell offset lx) in the lane's row
-    auto exact_at = [&](int lx, int vx, int u, int slot) -> bool {
-        const int kx = (u & 255) - 1 - lx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
-        if (kx < -1 || kx > 1 || ky < -1 || ky > 1 || kz < -1 || kz > 1) return false;
-        const double* f = pt_frac + 3 * (int64_t)cell_ids[slot];     // point-ordered FP64 offsets
-        const double zix = P.ax.zi[0][vx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
-        const double d1 = P.a[0] * (f[0] + (kx - zix));
-        const double d2 = P.a[1] * (f[1] + (ky - ziy));
-        const double d3 = P.a[2] * (f[2] + (kz - ziz));
-        return ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
-    };
-    auto kahan_add = [](float& s, float& c, float v) {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    };
-    float sre = 0.0f, sim = 0.0f, cre = 0.0f, cim = 0.0f;
-    float emax = -3.4028234663852886e38f;
-    int cnt = 0;
-    unsigned nguard = 0;
-    // ROW: sums and largest u of the row's four voxels over this lane's part
-    // of the candidates
-    float rre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float umax[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
-
-    int t_next = 0, kbase = 0;
-    while (t_next < T) {
-        // stage: compact candidates within 3 sigma of the tile box
-        int n = 0;
-        while (t_next < T && n <= CAP - 64) {
-            // cell of candidate t_next + lane
-            sflag[lane] = 0;
-            __syncthreads();
-            if (s_lo >= t_next && s_lo < t_next + 64) sflag[s_lo - t_next] = 1;
-            if (s_hi >= t_next && s_hi < t_next + 64) sflag[s_hi - t_next] = 1;
-            __syncthreads();
-            const unsigned long long sm = __ballot(sflag[lane] != 0);
-            const int k = kbase + below(sm) + ((sm >> lane) & 1ull ? 1 : 0) - 1;
-            kbase += (int)__popcll(sm);
-            const int t = t_next + lane;
-            bool keep = false;
-            float4 r0, r1, r2;
-            if (t < T) {
-                const float4 cr = crec[k];
-                const int slot = t + __float_as_int(cr.w);
-                const float4 pc = p32[slot];
-                const float px = pc.x + cr.x, py = pc.y + cr.y, pz = pc.z + cr.z;
-                const float ex = fmaxf(0.0f, fabsf(px) - hx);
-                const float ey = fmaxf(0.0f, fabsf(py) - hy);
-                const float ez = fmaxf(0.0f, fabsf(pz) - hz);
-                keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= F.rc_cull2;
-                if (keep) {
-                    const float m2k = -2.0f * k2;
-                    if (ROW) {
-                        float xv[4];
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const float d = px - qxt[i];
-                            xv[i] = __builtin_amdgcn_exp2f(k2 * (d * d));
-                        }
-                        r0 = make_float4(xv[0], xv[1], xv[2], xv[3]);
-                        r1 = make_float4(m2k * py, m2k * pz, k2 * fmaf(py, py, pz * pz), pc.w);
-                        r2 = make_float4(sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]), 0.0f);
-                    } else {
-                        r0 = make_float4(m2k * px, m2k * py, m2k * pz, k2 * fmaf(px, px, fmaf(py, py, pz * pz)));
-                        r1 = make_float4(pc.w, sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]));
-                    }
-                }
-            }
-            const unsigned long long m = __ballot(keep);
-            if (keep) {
-                const int p = n + below(m);
-                sA[REC * p] = r0;
-                sA[REC * p + 1] = r1;
-                if (ROW) sA[REC * p + 2] = r2;
-            }
-            n += (int)__popcll(m);
-            t_next += 64;
-        }
-        n = __builtin_amdgcn_readfirstlane(n);
-        if (ROW && lane < 4) {
-            // zero records after the last candidate: X = 0, never a member
-            // and never in the guard band (u = -1)
-            const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            sA[REC * (n + lane)] = z4;
-            sA[REC * (n + lane) + 1] = z4;
-            sA[REC * (n + lane) + 2] = z4;
-        }
-        __syncthreads();
-        if (DIAG) {
-            // every voxel against every staged candidate, per-pair exact
-            // membership with the counts and nearest ranges of the diagnostics
-            for (int jb = 0; jb < n; jb += 16) {
-                const int je = min(jb + 16, n);
-                float pr = 0.0f, pi = 0.0f;
-                for (int j = jb; j < je; j++) {
-                    const float4 A = sA[REC * j];
-                    const float4 C = sA[REC * j + 1];
-                    const float e = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2)));
-                    bool in = act && e >= F.e_rc;
-                    if (act && fabsf(e - F.e_rc) <= F.e_band) {
-                        nguard++;
-                        in = exact_at(lcx, jx, __float_as_int(C.w), __float_as_int(C.z));
-                    }
-                    if (F.thin_z) in = in && (unsigned)((__float_as_int(C.w) >> 16) - lcz) <= 2u;
-                    const float w = in ? __builtin_amdgcn_exp2f(e) : 0.0f;
-                    pr = fmaf(w, C.x, pr);
-                    pi = fmaf(w, C.y, pi);
-                    emax = in ? fmaxf(emax, e) : emax;
-                    cnt += in ? 1 : 0;
-                }
-                kahan_add(sre, cre, pr);
-                kahan_add(sim, cim, pi);
-            }
-        } else if (!ROW) {
-            // Branch-free pass over blocks of 32 candidates in the frame
-            // t = e - k2 (3 sigma)^2 (member iff t >= 0, weight 2^t scaled by
-            // 2^-e_rc), tracking the smallest |t|.  A block in which some
-            // lane came within the guard band is summed again with the exact
-            // FP64 test for those pairs (uniform branch, rare).
-            const float kq2r = kq2 - F.e_rc;
-            for (int jb = 0; jb < n; jb += 32) {
-                const int je = min(jb + 32, n);
-                float pr = 0.0f, pi = 0.0f, dmin = 3.4028234663852886e38f;
-#pragma unroll 4
-                for (int j = jb; j < je; j++) {
-                    const float4 A = sA[REC * j];
-                    const float4 C = sA[REC * j + 1];
-                    const float t = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2r)));
-                    const float w = t >= 0.0f ? __builtin_amdgcn_exp2f(t) : 0.0f;
-                    pr = fmaf(w, C.x, pr);
-                    pi = fmaf(w, C.y, pi);
-                    dmin = fminf(dmin, fabsf(t));
-                    if (DENSITY) emax = fmaxf(emax, t);
-                }
-                if (__ballot(dmin <= F.e_band)) {
-                    pr = 0.0f; pi = 0.0f;
-                    for (int j = jb; j < je; j++) {
-                        const float4 A = sA[REC * j];
-                        const float4 C = sA[REC * j + 1];
-                        const float t = fmaf(A.x, qcx, fmaf(A.y, qcy, fmaf(A.z, qcz, A.w + kq2r)));
-                        bool in = t >= 0.0f;
-                        if (fabsf(t) <= F.e_band) {
-                            nguard++;
-                            in = exact_at(lcx, jx, __float_as_int(C.w), __float_as_int(C.z));
-                        }
-                        const float w = in ? __builtin_amdgcn_exp2f(t) : 0.0f;
-                        pr = fmaf(w, C.x, pr);
-                        pi = fmaf(w, C.y, pi);
-                    }
-                }
-                kahan_add(sre, cre, pr);
-                kahan_add(sim, cim, pi);
-            }
-        } else {
-            // ROW: steps of NQ candidates (one per part), blocks of 8 steps;
-            // the guard band in u is the exponent's band (|u| ~ ln2 |t| near
-            // the threshold, so it is 1.44 times as wide).  The member weight
-            // is X E = u + 1, taken as w = s (u + 1) with the step
-            // s = clamp(2^40 u) in [0, 1]: 1 for u >= 2^-40, 0 for u <= 0, and
-            // fractional only inside the guard band, whose pairs the block's
-            // re-walk settles.
-            const int nst = (n + NQ - 1) / NQ;
-            auto step = [](float u) { return __builtin_amdgcn_fmed3f(u * 1099511627776.0f, 0.0f, 1.0f); };
-            for (int kb = 0; kb < nst; kb += 8) {
-                const int ke = min(kb + 8, nst);
-                float dmin = 3.4028234663852886e38f;
-#pragma unroll 2
-                for (int kk = kb; kk < ke; kk++) {
-                    const int j = NQ * kk + squart;
-                    const float4 X = sA[REC * j];
-                    const float4 B = sA[REC * j + 1];
-                    float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
-                    float sn;
-                    if (ROW == 2) {
-                        const float4 C = sA[REC * j + 2];
-                        sn = C.x;
-                        E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
-                    } else {
-                        sn = sA[REC * j + 2].x;
-                    }
-                    const float xv[4] = {X.x, X.y, X.z, X.w};
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const float u = fmaf(xv[i], E, -1.0f);
-                        const float s = step(u);
-                        const float w = fmaf(s, u, s);
-                        rre[i] = fmaf(w, B.w, rre[i]);
-                        rim[i] = fmaf(w, sn, rim[i]);
-                        dmin = fminf(dmin, fabsf(u));
-                        if (DENSITY) umax[i] = fmaxf(umax[i], u);
-                    }
-                }
-                if (__ballot(dmin <= F.e_band)) {
-                    for (int kk = kb; kk < ke; kk++) {
-                        const int j = NQ * kk + squart;
-                        const float4 X = sA[REC * j];
-                        const float4 B = sA[REC * j + 1];
-                        const float4 C = sA[REC * j + 2];
-                        float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
-                        if (ROW == 2) E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
-                        const float xv[4] = {X.x, X.y, X.z, X.w};
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const float u = fmaf(xv[i], E, -1.0f);
-                            if (fabsf(u) <= F.e_band) {
-                                const int vx = min(x0 + i, x1);
-                                if (row_act && x0 + i <= x1) nguard++;
-                                const bool in = exact_at(P.ax.ql[0][vx].lc, vx, __float_as_int(C.z),
-                                                         __float_as_int(C.y));
-                                // replace the pass's s (u + 1) by the exact test's weight
-                                const float s = step(u);
-                                const float d = (in ? u + 1.0f : 0.0f) - fmaf(s, u, s);
-                                rre[i] = fmaf(d, B.w, rre[i]);
-                                rim[i] = fmaf(d, C.x, rim[i]);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-
-    // the lane's voxels: ROW, voxels s NV .. s NV + NV - 1 of its row, from
-    // the parts' sums added in part order through LDS; otherwise its own
-    float vre[NV], vim[NV], vem[NV];
-    if (ROW) {
-        float2* red = reinterpret_cast<float2*>(sA);                  // [64 lanes][4 voxels]
-        float* redm = reinterpret_cast<float*>(sA) + 2 * 256;         // [64 lanes][4 voxels]
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            red[4 * lane + i] = make_float2(rre[i], rim[i]);
-            if (DENSITY) redm[4 * lane + i] = umax[i];
-        }
-        __syncthreads();
-        const int rb = lane & ~(NQ - 1);
-#pragma unroll
-        for (int t = 0; t < NV; t++) {
-            const int i = squart * NV + t;
-            float a = 0.0f, b = 0.0f, um = -1.0f;
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const float2 v = red[4 * (rb + q) + i];
-                a += v.x;
-                b += v.y;
-                if (DENSITY) um = fmaxf(um, redm[4 * (rb + q) + i]);
-            }
-            vre[t] = a;
-            vim[t] = b;
-            // largest exponent in the t frame: 2^t = u + 1 (u = -1: no candidate)
-            vem[t] = (DENSITY && um > -1.0f) ? __builtin_amdgcn_logf(um + 1.0f) : -3.4028234663852886e38f;
-        }
-    } else {
-        vre[0] = sre;
-        vim[0] = sim;
-        vem[0] = emax;
-    }
-
-#pragma unroll
-    for (int t = 0; t < NV; t++) {
-        const int vx = ROW ? x0 + squart * NV + t : ix;
-        if (!(ROW ? vx <= x1 && row_act : act)) continue;
-        const int64_t o = (int64_t)vx + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
-        const float sre_ = vre[t], sim_ = vim[t], emax_ = vem[t];
-        // x = S / max(|S|, 1e-16).  Without DIAG the sums carry the factor
-        // 2^-e_rc of the t frame (weights 2^t = 2^(e - e_rc)); it cancels in
-        // S/|S| and moves the threshold.
-        const float m2 = fmaf(sre_, sre_, sim_ * sim_);
-        const float ww = m2 > (DIAG ? 1.0e-32f : F.m2_min) ? __builtin_amdgcn_rsqf(m2)
-                                                          : (DIAG ? 1.0e16f : 1.0e16f * F.w_scale);
-        // nearest squared range from the largest exponent (huge(1.0) if none).
-        // Without DIAG the largest exponent is over every candidate, members
-        // or not: it differs from the members' only when no candidate is
-        // nearer than about 3 sigma, where the density is 0 either way.
-        float r2m;
-        if (DIAG) r2m = emax_ > -3.4028234663852886e38f ? emax_ / k2 : 3.4028234663852886e38f;
-        else r2m = emax_ > -3.4028234663852886e38f ? (emax_ + F.e_rc) * F.inv_k2 : 3.4028234663852886e38f;
-        float rh = 1.0f;
-        if (DENSITY && P.use_density) {
-            // convertScaledRangeToRho (Lib_DynamicalTwoBeamImaging.f90:1973-1994) in FP32
-            const float sg = (float)P.sigma;
-            if (r2m < sg * sg) rh = 1.0f;
-            else if (r2m > 4 * sg * sg) rh = 0.0f;
-            else {
-                const float xx = sqrtf(r2m) / sg - 1.0f;
-                rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
-            }
-        }
-        if (DENSITY) field[o] = make_float4(sre_ * ww, sim_ * ww, rh, 0.0f);
-        else reinterpret_cast<float2*>(field)[o] = make_float2(sre_ * ww, sim_ * ww);
-        if (DIAG) {
-            out_cnt[o] = cnt;
-            out_r2m[o] = r2m;
-        }
-    }
-    // spread over GUARD_SLOTS lines: many waves adding into one address
-    // serialise.  ROW lanes counted only their active voxels.
-    if ((ROW || act) && nguard)
-        atomicAdd(guard_count + GUARD_STRIDE * (blockIdx.x & (GUARD_SLOTS - 1)), (unsigned long long)nguard);
-}
-
-// ------------------------------------------------------------------ (c) integrate, FAST
-
-struct FastIntParams {
-    int Mx, My, Mz, np;         // np: precession points in this pass
-    float b0, bg, dz;
-    float aa[32];               // 2 pi s_g per precession point
-    float al[32];               // 2 dz aa (k_integrate_poly)
-};
-
-// One lane per pixel column, NP precession points marched together from
-// one read of the column (the field is shared by all precession points:
-// precession changes s_g only).  FP32 classic RK4 with h = 2 dz; the
-// per-step renormalisation of the reference is a positive scalar on a
-// linear map, so it is applied once at the end: I = |phi_g|^2/|phi|^2.
-// The mean-potential term c0 = b0 rho stays in, although in the exact ODE it
-// is a global phase: RK4's truncation error depends on the whole matrix, and
-// over 800 double steps leaving c0 out moves |phi_g|^2 by ~2e-4.
-template <int NP, bool DENS>
-__global__ __launch_bounds__(256) void k_integrate_fast(FastIntParams P, const void* __restrict__ field,
-                                                        double* __restrict__ img, int accumulate,
-                                                        double inv_total)
-{
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t plane = (int64_t)P.Mx * P.My;
-    if (col >= plane) return;
-    const int nz = P.Mz;
-    const float dz = P.dz, dz3 = P.dz / 3.0f, dz23 = 2.0f * P.dz / 3.0f, dz2 = 2.0f * P.dz;
-    float p0r[NP], p0i[NP], pgr[NP], pgi[NP];
-#pragma unroll
-    for (int p = 0; p < NP; p++) { p0r[p] = 1.0f; p0i[p] = 0.0f; pgr[p] = 0.0f; pgi[p] = 0.0f; }
-    // field: float4 {x, rho} with the density field, else float2 x (rho = 1);
-    // sample Mz is sample 0.  Samples are loaded one double-step ahead.
-    auto load = [&](int k) -> float4 {
-        const int64_t o = col + plane * (k == nz ? 0 : k);
-        // each sample is read once: non-temporal loads keep the stream from
-        // displacing cached data
-        typedef float ntf2 __attribute__((ext_vector_type(2)));
-        typedef float ntf4 __attribute__((ext_vector_type(4)));
-        if (DENS) {
-            const ntf4 v = __builtin_nontemporal_load(static_cast<const ntf4*>(field) + o);
-            return make_float4(v.x, v.y, v.z, v.w);
-        }
-        const ntf2 f = __builtin_nontemporal_load(static_cast<const ntf2*>(field) + o);
-        return make_float4(f.x, f.y, 1.0f, 0.0f);
-    };
-    auto coef = [&](const float4& f, float& c0, float& cgr, float& cgi) {
-        if (DENS) {
-            c0 = P.b0 * f.z;
-            const float bgr = P.bg * f.z;
-            cgr = bgr * f.x;
-            cgi = bgr * f.y;
-        } else {
-            c0 = P.b0;
-            cgr = P.bg * f.x;
-            cgi = P.bg * f.y;
-        }
-    };
-    // samples of the next PD double steps are in flight: ring[2u] and
-    // ring[2u + 1] hold samples ii + 1, ii + 2 of the double step at ii in
-    // slot u (ii = i0 + 2u), reloaded with those of ii + 2 PD once used.  A
-    // column is a chain of dependent loads, so the depth, not the bandwidth,
-    // bounds a wave's time when few points are marched per sample.
-    constexpr int PD = NP <= 4 ? 4 : 1;
-    float c0a = 0, cgra = 0, cgia = 0;
-    float4 ring[2 * PD];
-#pragma unroll
-    for (int k = 0; k < 2 * PD; k++) ring[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (nz >= 2) {
-        coef(load(0), c0a, cgra, cgia);
-#pragma unroll
-        for (int k = 0; k < 2 * PD; k++)
-            if (k + 1 <= nz) ring[k] = load(k + 1);
-    }
-    for (int i0 = 0; i0 <= nz - 2; i0 += 2 * PD) {
-#pragma unroll
-    for (int u = 0; u < PD; u++) {
-        const int ii = i0 + 2 * u;
-        if (ii > nz - 2) break;
-        float c0b, cgrb, cgib, c0c, cgrc, cgic;
-        coef(ring[2 * u], c0b, cgrb, cgib);
-        coef(ring[2 * u + 1], c0c, cgrc, cgic);
-        if (ii + 2 * PD + 2 <= nz) {
-            ring[2 * u] = load(ii + 2 * PD + 1);
-            ring[2 * u + 1] = load(ii + 2 * PD + 2);
-        }
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-            const float a = P.aa[p];
-            float d0r, d0i, dgr, dgi;
-#define FDERIV(c0, cgr, cgi)                                                   \
-    {                                                                          \
-        const float ca_ = c0 + a;                                              \
-        d0r = fmaf(-c0, p0i[p], fmaf(-cgi, pgr[p], -cgr * pgi[p]));           \
-        d0i = fmaf(c0, p0r[p], fmaf(cgr, pgr[p], -cgi * pgi[p]));             \
-        dgr = fmaf(cgi, p0r[p], fmaf(-cgr, p0i[p], -ca_ * pgi[p]));           \
-        dgi = fmaf(cgr, p0r[p], fmaf(cgi, p0i[p], ca_ * pgr[p]));             \
-    }
-            const float s0r = p0r[p], s0i = p0i[p], sgr = pgr[p], sgi = pgi[p];
-            FDERIV(c0a, cgra, cgia)
-            float a0r = fmaf(d0r, dz3, s0r), a0i = fmaf(d0i, dz3, s0i);
-            float agr = fmaf(dgr, dz3, sgr), agi = fmaf(dgi, dz3, sgi);
-            p0r[p] = fmaf(d0r, dz, s0r); p0i[p] = fmaf(d0i, dz, s0i);
-            pgr[p] = fmaf(dgr, dz, sgr); pgi[p] = fmaf(dgi, dz, sgi);
-            FDERIV(c0b, cgrb, cgib)
-            p0r[p] = fmaf(d0r, dz, s0r); p0i[p] = fmaf(d0i, dz, s0i);
-            pgr[p] = fmaf(dgr, dz, sgr); pgi[p] = fmaf(dgi, dz, sgi);
-            a0r = fmaf(d0r, dz23, a0r); a0i = fmaf(d0i, dz23, a0i);
-            agr = fmaf(dgr, dz23, agr); agi = fmaf(dgi, dz23, agi);
-            FDERIV(c0b, cgrb, cgib)
-            p0r[p] = fmaf(d0r, dz2, s0r); p0i[p] = fmaf(d0i, dz2, s0i);
-            pgr[p] = fmaf(dgr, dz2, sgr); pgi[p] = fmaf(dgi, dz2, sgi);
-            a0r = fmaf(d0r, dz23, a0r); a0i = fmaf(d0i, dz23, a0i);
-            agr = fmaf(dgr, dz23, agr); agi = fmaf(dgi, dz23, agi);
-            FDERIV(c0c, cgrc, cgic)
-#undef FDERIV
-            p0r[p] = fmaf(d0r, dz3, a0r); p0i[p] = fmaf(d0i, dz3, a0i);
-            pgr[p] = fmaf(dgr, dz3, agr); pgi[p] = fmaf(dgi, dz3, agi);
-        }
-        c0a = c0c; cgra = cgrc; cgia = cgic;
-    }
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        if (p >= P.np) break;
-        const float ng = fmaf(pgr[p], pgr[p], pgi[p] * pgi[p]);
-        const float n0 = fmaf(p0r[p], p0r[p], p0i[p] * p0i[p]);
-        acc += (double)ng / (double)(n0 + ng);
-    }
-    acc *= inv_total;
-    img[col] = accumulate ? img[col] + acc : acc;
-}
-
-// Many precession points.  The system is phi' = i H(a) phi with
-// H(a) = [[c0, cg], [conj(cg), c0 + a]] (the derivative above), so one RK4
-// double step is phi <- M phi with
-//     M = I + Q1/3 + (2/3) K2 + K3/3 + K4/6,
-//     Q1 = dz A_a,  T1 = I + Q1,  K2 = dz A_b T1,  T2 = I + K2,
-//     K3 = 2 dz A_b T2,  T3 = I + K3,  K4 = 2 dz A_c T3,   A_x = i H_x(a),
-// the same map as the stage-by-stage march (A_a, A_b, A_c at the double
-// step's three samples).  a enters only through the (2,2) element, once per
-// stage, so the entries of M are polynomials in alpha = 2 dz a of degrees 2,
-// 3, 3 and 4 whose coefficients depend on the column's field samples only.
-// They are formed once per double step and shared: a precession point then
-// costs a Horner evaluation at its alpha and one matrix-vector product
-// (~40 FMAs) instead of four derivative evaluations and the stage updates
-// (~80).  Results differ from k_integrate_fast by rounding only.
-struct cf {
-    float r, i;
-};
-template <int D>
-struct CPoly {
-    cf c[D + 1];    // coefficients of alpha^0 .. alpha^D
-};
-
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-    if constexpr (B <= E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-// One column (P1, P2) of Q = s A_x P for a polynomial matrix P, with the
-// stage's scaled coefficients c = s c0, g = s cg and ax = s / (2 dz):
-//     Q1 = i (c P1 + g P2),   Q2 = i (conj(g) P1 + c P2 + ax alpha P2).
-// Terms are chained from the first present one, so structural zeros cost
-// nothing.
-template <int D1, int D2, int E1 = (D1 > D2 ? D1 : D2), int E2 = (D1 > D2 + 1 ? D1 : D2 + 1)>
-__device__ __forceinline__ void rk_col(float c, float gr, float gi, float ax, const CPoly<D1>& P1, const CPoly<D2>& P2,
-                                       CPoly<E1>& Q1, CPoly<E2>& Q2)
-{
-    static_for<0, E1>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        float ur, ui;
-        if constexpr (k <= D2) {
-            ur = fmaf(gr, P2.c[k].r, -gi * P2.c[k].i);
-            ui = fmaf(gr, P2.c[k].i, gi * P2.c[k].r);
-            if constexpr (k <= D1) {
-                ur = fmaf(c, P1.c[k].r, ur);
-                ui = fmaf(c, P1.c[k].i, ui);
-            }
-        } else {
-            ur = c * P1.c[k].r;
-            ui = c * P1.c[k].i;
-        }
-        Q1.c[k] = cf{-ui, ur};
-    });
-    static_for<0, E2>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        constexpr bool ta = k <= D1, tb = k <= D2, tc = k >= 1 && k - 1 <= D2;
-        float ur = 0.0f, ui = 0.0f;
-        if constexpr (ta) {
-            ur = fmaf(gr, P1.c[k].r, gi * P1.c[k].i);
-            ui = fmaf(gr, P1.c[k].i, -gi * P1.c[k].r);
-        }
-        if constexpr (tb) {
-            if constexpr (ta) {
-                ur = fmaf(c, P2.c[k].r, ur);
-                ui = fmaf(c, P2.c[k].i, ui);
-            } else {
-                ur = c * P2.c[k].r;
-                ui = c * P2.c[k].i;
-            }
-        }
-        if constexpr (tc) {
-            if constexpr (ta || tb) {
-                ur = fmaf(ax, P2.c[k - 1].r, ur);
-                ui = fmaf(ax, P2.c[k - 1].i, ui);
-            } else {
-                ur = ax * P2.c[k - 1].r;
-                ui = ax * P2.c[k - 1].i;
-            }
-        }
-        Q2.c[k] = cf{-ui, ur};
-    });
-}
-
-// M += s P (degree of P <= degree of M)
-template <int DM, int DP>
-__device__ __forceinline__ void poly_axpy(CPoly<DM>& M, float s, const CPoly<DP>& P)
-{
-    static_assert(DP <= DM, "degree");
-#pragma unroll
-    for (int k = 0; k <= DP; k++) {
-        M.c[k].r = fmaf(s, P.c[k].r, M.c[k].r);
-        M.c[k].i = fmaf(s, P.c[k].i, M.c[k].i);
-    }
-}
-
-template <int D>
-__device__ __forceinline__ cf horner(const CPoly<D>& M, float al)
-{
-    cf v = M.c[D];
-#pragma unroll
-    for (int k = D - 1; k >= 0; k--) {
-        v.r = fmaf(v.r, al, M.c[k].r);
-        v.i = fmaf(v.i, al, M.c[k].i);
-    }
-    return v;
-}
-
-template <int NP, bool DENS>
-__global__ __launch_bounds__(256) void k_integrate_poly(FastIntParams P, const void* __restrict__ field,
-                                                        double* __restrict__ img, int accumulate,
-                                                        double inv_total)
-{
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t plane = (int64_t)P.Mx * P.My;
-    if (col >= plane) return;
-    const int nz = P.Mz;
-    const float dz = P.dz, dz2 = 2.0f * P.dz, dz6 = P.dz / 3.0f;
-    float p0r[NP], p0i[NP], pgr[NP], pgi[NP];
-#pragma unroll
-    for (int p = 0; p < NP; p++) { p0r[p] = 1.0f; p0i[p] = 0.0f; pgr[p] = 0.0f; pgi[p] = 0.0f; }
-    auto load = [&](int k) -> float4 {
-        const int64_t o = col + plane * (k == nz ? 0 : k);
-        if (DENS) return static_cast<const float4*>(field)[o];
-        const float2 f = static_cast<const float2*>(field)[o];
-        return make_float4(f.x, f.y, 1.0f, 0.0f);
-    };
-    auto coef = [&](const float4& f, float& c0, float& cgr, float& cgi) {
-        if (DENS) {
-            c0 = P.b0 * f.z;
-            const float bgr = P.bg * f.z;
-            cgr = bgr * f.x;
-            cgi = bgr * f.y;
-        } else {
-            c0 = P.b0;
-            cgr = P.bg * f.x;
-            cgi = P.bg * f.y;
-        }
-    };
-    float c0a = 0, cgra = 0, cgia = 0;
-    float4 fb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), fc = fb;
-    if (nz >= 2) {
-        coef(load(0), c0a, cgra, cgia);
-        fb = load(1);
-        fc = load(2);
-    }
-    for (int ii = 0; ii <= nz - 2; ii += 2) {
-        float c0b, cgrb, cgib, c0c, cgrc, cgic;
-        coef(fb, c0b, cgrb, cgib);
-        coef(fc, c0c, cgrc, cgic);
-        if (ii + 4 <= nz) {
-            fb = load(ii + 3);
-            fc = load(ii + 4);
-        }
-        // T1 = I + dz A_a
-        const float ca = dz * c0a, gar = dz * cgra, gai = dz * cgia;
-        CPoly<0> T11{{cf{1.0f, ca}}}, T12{{cf{-gai, gar}}}, T21{{cf{gai, gar}}};
-        CPoly<1> T22{{cf{1.0f, ca}, cf{0.0f, 0.5f}}};
-        // K2 = dz A_b T1, T2 = I + K2
-        CPoly<0> K2_11; CPoly<1> K2_21, K2_12; CPoly<2> K2_22;
-        rk_col(dz * c0b, dz * cgrb, dz * cgib, 0.5f, T11, T21, K2_11, K2_21);
-        rk_col(dz * c0b, dz * cgrb, dz * cgib, 0.5f, T12, T22, K2_12, K2_22);
-        CPoly<0> U11 = K2_11; CPoly<1> U21 = K2_21, U12 = K2_12; CPoly<2> U22 = K2_22;
-        U11.c[0].r += 1.0f;
-        U22.c[0].r += 1.0f;
-        // K3 = 2 dz A_b T2, T3 = I + K3
-        CPoly<1> K3_11; CPoly<2> K3_21, K3_12; CPoly<3> K3_22;
-        rk_col(dz2 * c0b, dz2 * cgrb, dz2 * cgib, 1.0f, U11, U21, K3_11, K3_21);
-        rk_col(dz2 * c0b, dz2 * cgrb, dz2 * cgib, 1.0f, U12, U22, K3_12, K3_22);
-        CPoly<1> V11 = K3_11; CPoly<2> V21 = K3_21, V12 = K3_12; CPoly<3> V22 = K3_22;
-        V11.c[0].r += 1.0f;
-        V22.c[0].r += 1.0f;
-        // K4/6 = (dz/3) A_c T3: the last term of M, which it starts
-        CPoly<2> M11; CPoly<3> M21, M12; CPoly<4> M22;
-        rk_col(dz6 * c0c, dz6 * cgrc, dz6 * cgic, 1.0f / 6.0f, V11, V21, M11, M21);
-        rk_col(dz6 * c0c, dz6 * cgrc, dz6 * cgic, 1.0f / 6.0f, V12, V22, M12, M22);
-        // + K3/3 + (2/3) K2 + Q1/3 + I
-        poly_axpy(M11, 1.0f / 3.0f, K3_11); poly_axpy(M21, 1.0f / 3.0f, K3_21);
-        poly_axpy(M12, 1.0f / 3.0f, K3_12); poly_axpy(M22, 1.0f / 3.0f, K3_22);
-        poly_axpy(M11, 2.0f / 3.0f, K2_11); poly_axpy(M21, 2.0f / 3.0f, K2_21);
-        poly_axpy(M12, 2.0f / 3.0f, K2_12); poly_axpy(M22, 2.0f / 3.0f, K2_22);
-        M11.c[0].r += 1.0f;
-        M11.c[0].i = fmaf(ca, 1.0f / 3.0f, M11.c[0].i);
-        M12.c[0].r = fmaf(-gai, 1.0f / 3.0f, M12.c[0].r);
-        M12.c[0].i = fmaf(gar, 1.0f / 3.0f, M12.c[0].i);
-        M21.c[0].r = fmaf(gai, 1.0f / 3.0f, M21.c[0].r);
-        M21.c[0].i = fmaf(gar, 1.0f / 3.0f, M21.c[0].i);
-        M22.c[0].r += 1.0f;
-        M22.c[0].i = fmaf(ca, 1.0f / 3.0f, M22.c[0].i);
-        M22.c[1].i += 1.0f / 6.0f;
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-            const float al = P.al[p];
-            const cf m11 = horner(M11, al), m12 = horner(M12, al), m21 = horner(M21, al), m22 = horner(M22, al);
-            const float y0r = p0r[p], y0i = p0i[p], ygr = pgr[p], ygi = pgi[p];
-            p0r[p] = fmaf(m11.r, y0r, fmaf(-m11.i, y0i, fmaf(m12.r, ygr, -m12.i * ygi)));
-            p0i[p] = fmaf(m11.r, y0i, fmaf(m11.i, y0r, fmaf(m12.r, ygi, m12.i * ygr)));
-            pgr[p] = fmaf(m21.r, y0r, fmaf(-m21.i, y0i, fmaf(m22.r, ygr, -m22.i * ygi)));
-            pgi[p] = fmaf(m21.r, y0i, fmaf(m21.i, y0r, fmaf(m22.r, ygi, m22.i * ygr)));
-        }
-        c0a = c0c; cgra = cgrc; cgia = cgic;
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        if (p >= P.np) break;
-        const float ng = fmaf(pgr[p], pgr[p], pgi[p] * pgi[p]);
-        const float n0 = fmaf(p0r[p], p0r[p], p0i[p] * p0i[p]);
-        acc += (double)ng / (double)(n0 + ng);
-    }
-    acc *= inv_total;
-    img[col] = accumulate ? img[col] + acc : acc;
-}
-
-// ------------------------------------------------------------------ (d) blur
-
-struct BlurParams {
-    int nx, ny, nk;
-};
-
-static constexpr int BLUR_TILE = 256;
-static constexpr int BLUR_MAXK = 256;
-
-// x pass of gaussianBlurImage (d2bi.f90:1577-1599): row segments staged in
-// LDS with their halo; the kernel weights come from the host (exp in FP64).
-__global__ __launch_bounds__(BLUR_TILE) void k_blur_x(BlurParams B, const double* __restrict__ fin,
-                                                      const double* __restrict__ kern,
-                                                      double* __restrict__ tmp, double* __restrict__ wsx)
-{
-    __shared__ double row[BLUR_TILE + 2 * BLUR_MAXK];
-    const int iy = blockIdx.y;
-    const int x0 = blockIdx.x * BLUR_TILE;
-    const int lo = x0 - B.nk, n = BLUR_TILE + 2 * B.nk;
-    for (int t = threadIdx.x; t < n; t += BLUR_TILE) {
-        const int gx = lo + t;
-        row[t] = (gx >= 0 && gx < B.nx) ? fin[(int64_t)gx + (int64_t)B.nx * iy] : 0.0;
-    }
-    __syncthreads();
-    const int ix = x0 + threadIdx.x;
-    if (ix >= B.nx) return;
-    const int j0 = max(0, ix - B.nk), j1 = min(B.nx - 1, ix + B.nk);
-    double wf = 0.0, ws = 0.0;
-    for (int jj = j0; jj <= j1; jj++) {
-        const double ww = kern[abs(jj - ix)];
-        wf = wf + ww * row[jj - lo];
-        ws = ws + ww;
-    }
-    tmp[(int64_t)ix + (int64_t)B.nx * iy] = wf;
-    if (iy == 0) wsx[ix] = 1 / ws;
-}
-
-// y pass (d2bi.f90:1605-1619): out = sum_y k * tmp * (1/W_x) / W_y.
-__global__ __launch_bounds__(256) void k_blur_y(BlurParams B, const double* __restrict__ tmp,
-                                                const double* __restrict__ kern,
-                                                const double* __restrict__ wsx, double* __restrict__ out)
-{
-    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (ix >= B.nx || iy >= B.ny) return;
-    const int j0 = max(0, iy - B.nk), j1 = min(B.ny - 1, iy + B.nk);
-    double wf = 0.0, ws = 0.0;
-    for (int jj = j0; jj <= j1; jj++) {
-        const double ww = kern[abs(jj - iy)];
-        wf = wf + ww * tmp[(int64_t)ix + (int64_t)B.nx * jj];
-        ws = ws + ww;
-    }
-    out[(int64_t)ix + (int64_t)B.nx * iy] = (wf * wsx[ix]) / ws;
-}
-
-// ------------------------------------------------------------------ generic link-cell query
-
-struct QueryParams {
-    double a[9], ia[9];
-    int Nx, Ny, Nz;
-    double rc2;
-    int maxNeigh;
-};
-
-// neighbourList1b (Lib_LinkCell3D.f90:1515-1605) for arbitrary query points
-// and a general (not necessarily cuboidal) cell.
-__global__ void k_lc_query(QueryParams Q, int64_t nq, const double* __restrict__ q,
-                           const int* __restrict__ cell_start, const int* __restrict__ slot_id,
-                           const double4* __restrict__ frac, int* __restrict__ nout,
-                           int* __restrict__ idout, double* __restrict__ r2out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const double x[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
-    double zi[3];
-    for (int k = 0; k < 3; k++) zi[k] = (Q.ia[k] * x[0] + Q.ia[3 + k] * x[1]) + Q.ia[6 + k] * x[2];
-    const int ix = cell_of(floor(zi[0]), Q.Nx);
-    const int iy = cell_of(floor(zi[1]), Q.Ny);
-    const int iz = cell_of(floor(zi[2]), Q.Nz);
-    zi[0] = zi[0] - ix; zi[1] = zi[1] - iy; zi[2] = zi[2] - iz;
-    int n = 0;
-    for (int kz = -1; kz <= 1; kz++) {
-        const int jz = (iz + Q.Nz + kz) % Q.Nz;
-        const double dzi3 = kz - zi[2];
-        for (int ky = -1; ky <= 1; ky++) {
-            const int jy = (iy + Q.Ny + ky) % Q.Ny;
-            const double dzi2 = ky - zi[1];
-            for (int kx = -1; kx <= 1; kx++) {
-                const int jx = (ix + Q.Nx + kx) % Q.Nx;
-                const double dzi1 = kx - zi[0];
-                const int c = jx + Q.Nx * (jy + Q.Ny * jz);
-                for (int s = cell_start[c]; s < cell_start[c + 1]; s++) {
-                    const double4 f = frac[s];
-                    const double dzj1 = f.x + dzi1, dzj2 = f.y + dzi2, dzj3 = f.z + dzi3;
-                    double d[3];
-                    for (int k = 0; k < 3; k++) d[k] = (Q.a[k] * dzj1 + Q.a[3 + k] * dzj2) + Q.a[6 + k] * dzj3;
-                    const double dd = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-                    if (dd <= Q.rc2) {
-                        if (n < Q.maxNeigh) {
-                            idout[i * Q.maxNeigh + n] = slot_id[s];
-                            r2out[i * Q.maxNeigh + n] = dd;
-                        }
-                        n++;
-                    }
-                }
-            }
-        }
-    }
-    nout[i] = n;
-}
-
-// ------------------------------------------------------------------ orchestration
-
-struct Timer {
-    hipEvent_t* ev;
-    hipStream_t s;
-    int record(int i)
-    {
-        HIPCHK(hipEventRecord(ev[i], s));
-        return D2BI_OK;
-    }
-    double ms(int a, int b)
-    {
-        float t = 0.0f;
-        (void)hipEventElapsedTime(&t, ev[a], ev[b]);
-        return t;
-    }
-};
-
-static void fill_vox_params(const d2bi_record* r, const d2bi_plan* p, const AxisDev& ax, VoxParams* V)
-{
-    V->Mx = p->Mx; V->My = p->My; V->Mz = p->Mz;
-    V->Ex = p->Ex; V->Ey = p->Ey; V->Ez = p->Ez;
-    V->a[0] = p->a_col[0]; V->a[1] = p->a_col[4]; V->a[2] = p->a_col[8];
-    V->rc2 = p->rc2; V->i2s2 = p->i2s2; V->sigma = p->sigma;
-    V->use_density = r->use_density ? 1 : 0;
-    V->ax = ax;
-}
-
-// Tile depth of a FAST kernel variant: 0 (a lane per voxel) and 1 (rows of
-// 4 x 4 x 4 tiles) use 4 voxels, 2 (rows of 4 x 4 x 8 tiles) 8.
-static int variant_tz(int variant) { return variant == 2 ? 8 : 4; }
-
-// Link cells in the largest unwrapped cell range of a 4 x 4 x tz tile.
-static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz)
-{
-    int worst = 1;
-    for (int a = 0; a < 3; a++) {
-        const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
-        const int ts = a == 2 ? tz : TILE;
-        int mx = 0;
-        for (int t0 = 0; t0 < M; t0 += ts)
-            mx = std::max(mx, tab.cell[a][std::min(M, t0 + ts) - 1] - tab.cell[a][t0] + 3);
-        worst *= mx;
-    }
-    return worst;
-}
-
-// Launch parameters of the FAST voxel kernel for a plan, tiles of 4 x 4 x tz.
-static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo, int tz)
-{
-    FastParams& F = *Fo;
-    F.v = V;
-    F.tz = tz;
-    F.ntx = (p->Mx + TILE - 1) / TILE;
-    F.nty = (p->My + TILE - 1) / TILE;
-    F.ntz = (p->Mz + tz - 1) / tz;
-    F.ntiles = F.ntx * F.nty * F.ntz;
-    F.per_xcd = (F.ntiles + 7) / 8;
-    // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
-    // rounded once from FP64, three FMAs); +-1e-5 relative covers it 5x
-    const double k2 = -p->i2s2 * 1.4426950408889634;
-    F.k2 = (float)k2;
-    F.e_rc = (float)(k2 * p->rc2);
-    F.e_band = (float)(std::fabs(k2 * p->rc2) * 1.0e-5);
-    F.w_scale = (float)std::exp2((double)F.e_rc);
-    F.m2_min = (float)(1.0e-32 / ((double)F.w_scale * (double)F.w_scale));
-    F.inv_k2 = (float)(1.0 / k2);
-    F.rc_cull2 = (float)(p->rc2 * (1.0 + 1.0e-4));
-    F.cellf[0] = (float)p->a_col[0]; F.cellf[1] = (float)p->a_col[4]; F.cellf[2] = (float)p->a_col[8];
-    // cells thinner than 3 sigma by more than rounding (the guard band, ~2e-5
-    // A, covers less): only z can be, x and y are wide by construction
-    const double rc = std::sqrt(p->rc2), thin = rc * (1.0 - 1.0e-7);
-    if (p->a_col[0] < thin || p->a_col[4] < thin)
-        return seterr(D2BI_ERR_ARG, "link cells %.6f x %.6f narrower than 3 sigma = %.6f", p->a_col[0],
-                      p->a_col[4], rc);
-    F.thin_z = p->a_col[8] < thin ? 1 : 0;
-    // the unwrapped cell range of a tile must fit the per-tile cell table
-    const int worst = tile_cells(p, tab, tz);
-    if (worst > TILE_MAXC) return seterr(D2BI_ERR_CAPACITY, "voxel tile spans %d link cells (max %d)", worst, TILE_MAXC);
-    return D2BI_OK;
-}
-
-// Launch parameters of a variant.  A 4 x 4 x 8 tile that spans more link
-// cells than the per-tile table holds falls back to the 4 x 4 x 4 row kernel
-// (*variant becomes 1).
-static int variant_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, int* variant,
-                          FastParams* F)
-{
-    if (*variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) *variant = 1;
-    return fast_params(p, tab, V, F, variant_tz(*variant));
-}
-
-// Kernel instance for (diagnostics, density, variant); link cells thinner
-// than 3 sigma along z take a row kernel (variant 1 or 2) with the z-offset
-// test.
-template <bool DENS, int ROW, class Go>
-static void pick_row(int tz, Go go)
-{
-    if (tz == 8) go(k_voxel_tile<DENS, false, ROW, 8>);
-    else go(k_voxel_tile<DENS, false, ROW, 4>);
-}
-
-template <class Go>
-static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
-{
-    if (diag) { go(k_voxel_tile<true, true, 0, 4>); return; }
-    const int tz = variant_tz(variant);
-    if (!thin_z && variant == 0) {
-        if (dens) go(k_voxel_tile<true, false, 0, 4>);
-        else go(k_voxel_tile<false, false, 0, 4>);
-    } else if (thin_z) {
-        if (dens) pick_row<true, 2>(tz, go);
-        else pick_row<false, 2>(tz, go);
-    } else {
-        if (dens) pick_row<true, 1>(tz, go);
-        else pick_row<false, 1>(tz, go);
-    }
-}
-
-template <bool DIAG>
-static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start, const int* cell_ids,
-                        const double* pt_frac, const float4* p32, const float* sn32, float4* field, int* cnt,
-                        float* r2m, unsigned long long* gc, hipStream_t s)
-{
-    pick_tile(DIAG, dens, variant, F.thin_z != 0, [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, cell_ids, pt_frac, p32, sn32,
-                           field, cnt, r2m, gc);
-    });
-}
-
-// Choice of the FAST voxel kernel variant (0: a lane per voxel, 1: a lane
-// per row of four voxels and quarter of the candidates of a 4 x 4 x 4 tile,
-// 2: a lane per row and half of the candidates of a 4 x 4 x 8 tile) per
-// voxel-grid shape: timed once on the first condition of that shape, then
-// cached.  D2BI_FAST_VARIANT=0|1|2 forces one.
-static std::map<std::array<int, 7>, int> g_variant_cache;
-
-static int forced_variant()
-{
-    const char* env = std::getenv("D2BI_FAST_VARIANT");
-    if (env && env[0] >= '0' && env[0] <= '2' && env[1] == 0) return env[0] - '0';
-    return -1;
-}
-
-// cached (or forced) variant for a shape, -1 if not tuned yet
-static int known_variant(const d2bi_plan* p, bool dens)
-{
-    const int f = forced_variant();
-    if (f >= 0) return f;
-    const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
-    auto it = g_variant_cache.find(key);
-    return it == g_variant_cache.end() ? -1 : it->second;
-}
-
-template <class TimeFn>
-static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, TimeFn time_variant)
-{
-    auto& cache = g_variant_cache;
-    const int f = forced_variant();
-    if (f >= 0) return f;
-    if (diag) return 0;
-    const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    // each variant twice, the second (warm-cache) timing counts; thin z
-    // cells have only the row kernels
-    const int v0 = thin_z ? 1 : 0;
-    float t[3] = {0.0f, 0.0f, 0.0f};
-    for (int rep = 0; rep < 2; rep++)
-        for (int v = v0; v < 3; v++) {
-            t[v] = time_variant(v);
-            if (t[v] < 0) return seterr(D2BI_ERR_HIP, "variant timing failed");
-        }
-    int best = v0;
-    for (int v = v0 + 1; v < 3; v++)
-        if (t[v] < t[best]) best = v;
-    cache[key] = best;
-    return best;
-}
-
-// Voxelise into the mode's field buffers; optional count / r2min outputs.
-static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables& tab, const AxisDev& ax,
-                     int mode, bool want_diag, int64_t* n_guard, hipStream_t s, Timer* T = nullptr)
-{
-    Workspace& w = g.ws;
-    const int64_t nvox = (int64_t)p->Mx * p->My * p->Mz;
-    VoxParams V;
-    fill_vox_params(r, p, ax, &V);
-    int* dcnt = nullptr;
-    void* dr2 = nullptr;
-    if (want_diag) {
-        CHK(w.cnt.ensure(sizeof(int) * nvox));
-        CHK(w.r2m.ensure(sizeof(double) * nvox));
-        dcnt = w.cnt.as<int>();
-        dr2 = w.r2m.p;
-    }
-    if (mode == D2BI_MODE_FP64) {
-        CHK(w.f64_re.ensure(sizeof(double) * nvox));
-        CHK(w.f64_im.ensure(sizeof(double) * nvox));
-        CHK(w.f64_rho.ensure(sizeof(double) * nvox));
-        dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
-        if (T) CHK(T->record(2));
-        hipLaunchKernelGGL(k_voxel_fp64, grid, dim3(256), 0, s, V, w.cell_start.as<int>(),
-                           w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), w.f64_re.as<double>(),
-                           w.f64_im.as<double>(), w.f64_rho.as<double>(), dcnt, (double*)dr2);
-        HIPCHK(hipGetLastError());
-        if (T) CHK(T->record(3));
-        return D2BI_OK;
-    }
-    CHK(w.f32_field.ensure(sizeof(float4) * nvox));
-    FastParams F;
-    CHK(fast_params(p, tab, V, &F, 4));
-    CHK(w.counters.ensure(64));
-    unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
-    const size_t gp_bytes = sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE;
-    CHK(w.guard_part.ensure(gp_bytes));
-    unsigned long long* gp = w.guard_part.as<unsigned long long>();
-    const bool dens = r->use_density != 0;
-    // launch parameters follow the variant's tile depth
-    auto run_variant = [&](int v) -> int {
-        FastParams Fv;
-        CHK(variant_params(p, tab, V, &v, &Fv));
-        pick_tile(want_diag, dens, v, Fv.thin_z != 0, [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(8 * Fv.per_xcd), dim3(64), 0, s, Fv, w.cell_start.as<int>(),
-                               w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
-                               w.slot_sn32.as<float>(), w.f32_field.as<float4>(), dcnt, (float*)dr2, gp);
-        });
-        return D2BI_OK;
-    };
-    const int variant = fast_variant(p, dens, want_diag, F.thin_z != 0, [&](int v) -> float {
-        // a variant whose tiles span too many link cells is not timed
-        if (tile_cells(p, tab, variant_tz(v)) > TILE_MAXC) return 3.4e38f;
-        HIPCHK(hipEventRecord(g.ev[7], s));
-        CHK(run_variant(v));
-        HIPCHK(hipEventRecord(g.ev[8], s));
-        HIPCHK(hipEventSynchronize(g.ev[8]));
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, g.ev[7], g.ev[8]);
-        return ms;
-    });
-    if (variant < 0) return variant;
-    HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches counted too
-    if (T) CHK(T->record(2));
-    CHK(run_variant(variant));
-    HIPCHK(hipGetLastError());
-    if (T) CHK(T->record(3));
-    hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
-    if (n_guard) {
-        unsigned long long v = 0;
-        HIPCHK(hipMemcpyAsync(&v, gc, 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        *n_guard = (int64_t)v;
-    }
-    return D2BI_OK;
-}
-
-// FAST integration of 8 or more precession points per pass through the
-// shared step matrix (k_integrate_poly); D2BI_FAST_INTEGRATOR=stage keeps the
-// stage-by-stage march for every pass.
-static bool fast_integrator_poly()
-{
-    const char* e = std::getenv("D2BI_FAST_INTEGRATOR");
-    return !(e && std::strcmp(e, "stage") == 0);
-}
-
-template <int NP>
-static void launch_int_fast(const FastIntParams& P, bool dens, const void* field, double* img, int acc,
-                            double inv, hipStream_t s)
-{
-    const int64_t plane = (int64_t)P.Mx * P.My;
-    const dim3 grid((unsigned)((plane + 255) / 256));
-    // from 8 points on, the shared step matrix costs less than it saves
-    if constexpr (NP >= 8) {
-        if (fast_integrator_poly()) {
-            if (dens) hipLaunchKernelGGL((k_integrate_poly<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
-            else hipLaunchKernelGGL((k_integrate_poly<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
-            return;
-        }
-    }
-    if (dens) hipLaunchKernelGGL((k_integrate_fast<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
-    else hipLaunchKernelGGL((k_integrate_fast<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
-}
-
-// FAST integration of all precession points, in passes of up to 32.
-// dens: the field holds float4 {x, rho} (density field on), else float2 x.
-// acc_in: add to d_img from the first pass on; inv: weight of one
-// integration (0: 1/nPrec).
-static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, double* d_img, hipStream_t s,
-                                 bool acc_in = false, double inv = 0.0)
-{
-    if (inv == 0.0) inv = 1.0 / p->nPrec;
-    for (int p0 = 0, first = acc_in ? 0 : 1; p0 < p->nPrec; first = 0) {
-        const int left = p->nPrec - p0;
-        const int np = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
-        FastIntParams P;
-        P.Mx = p->Mx; P.My = p->My; P.Mz = p->Mz; P.np = np;
-        P.b0 = (float)p->b0; P.bg = (float)p->bg; P.dz = (float)p->dz;
-        for (int i = 0; i < 32; i++) {
-            P.aa[i] = i < np ? (float)p->aa[p0 + i] : 0.0f;
-            P.al[i] = i < np ? (float)(2.0 * p->dz * p->aa[p0 + i]) : 0.0f;
-        }
-        switch (np) {
-            case 32: launch_int_fast<32>(P, dens, f, d_img, !first, inv, s); break;
-            case 16: launch_int_fast<16>(P, dens, f, d_img, !first, inv, s); break;
-            case 8: launch_int_fast<8>(P, dens, f, d_img, !first, inv, s); break;
-            case 4: launch_int_fast<4>(P, dens, f, d_img, !first, inv, s); break;
-            case 2: launch_int_fast<2>(P, dens, f, d_img, !first, inv, s); break;
-            default: launch_int_fast<1>(P, dens, f, d_img, !first, inv, s); break;
-        }
-        HIPCHK(hipGetLastError());
-        p0 += np;
-    }
-    return D2BI_OK;
-}
-
-// accumulate: add each column's sum over precession points to d_img;
-// otherwise write sum / nPrec (div = 0) or sum / div.
-static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double* d_img, hipStream_t s,
-                                 int accumulate = 0, double div = 0.0)
-{
-    Workspace& w = g.ws;
-    const int64_t plane = (int64_t)p->Mx * p->My;
-    IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa, accumulate,
-                div == 0.0 ? (double)p->nPrec : div};
-    hipLaunchKernelGGL(k_integrate_fp64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
-                       w.f64_re.as<double>(), w.f64_im.as<double>(), w.f64_rho.as<double>(), d_img);
-    HIPCHK(hipGetLastError());
-    return D2BI_OK;
-}
-
-static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s, Timer* T = nullptr)
-{
-    Workspace& w = g.ws;
-    const int64_t plane = (int64_t)p->Mx * p->My;
-    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
-    double* d_img = w.img.as<double>();
-    if (mode == D2BI_MODE_FP64) {
-        double* d_aa = d_img + plane;
-        HIPCHK(hipMemcpyAsync(d_aa, p->aa, sizeof(double) * p->nPrec, hipMemcpyHostToDevice, s));
-        if (T) CHK(T->record(4));
-        CHK(launch_integrate_fp64(p, d_aa, d_img, s));
-        if (T) CHK(T->record(5));
-        // the aa copy reads p->aa (caller memory): finish before returning
-        HIPCHK(hipStreamSynchronize(s));
-        return D2BI_OK;
-    }
-    if (T) CHK(T->record(4));
-    CHK(launch_integrate_fast(p, dens, w.f32_field.p, d_img, s));
-    if (T) CHK(T->record(5));
-    return D2BI_OK;
-}
-
-// Whole condition: bin -> voxelise -> integrate, result left in ws.img.
-static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
-                           int mode, d2bi_stats* st, d2bi_plan* plan_out)
-{
-    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
-    d2bi_plan plan;
-    CHK(plan_image(r, n, precAngle, nPrec, &plan));
-    AxisTables tab;
-    CHK(axis_tables(r, &plan, &tab));
-    BinParams BP;
-    CHK(bin_params(r, &plan, &BP));
-    BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
-    hipStream_t s = g.stream;
-    Timer T{g.ev, s};
-    CHK(T.record(0));
-    AxisDev ax;
-    CHK(upload_axis(tab, &ax, s));
-    Binned b;
-    CHK(run_binning(d_x, n, BP, &b, s));
-    CHK(T.record(1));
-    CHK(run_voxel(r, &plan, tab, ax, mode, false, nullptr, s, &T));
-    CHK(run_integrate(&plan, r->use_density != 0, mode, s, &T));
-    CHK(T.record(6));
-    // counters (largest cell count at 0, FAST guard re-tests at 8) come back
-    // with the final synchronisation
-    int64_t* hc = nullptr;
-    if (st) {
-        void* hp;
-        CHK(pinned(16, &hp));
-        hc = (int64_t*)hp;
-        HIPCHK(hipMemcpyAsync(hc, g.ws.counters.p, 16, hipMemcpyDeviceToHost, s));
-    }
-    HIPCHK(hipStreamSynchronize(s));
-    if (st) {
-        const int mpc = (int)(hc[0] & 0xffffffff);
-        st->n_points += b.nr;
-        st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
-        st->max_per_cell = std::max(st->max_per_cell, mpc);
-        st->n_cells = b.ncell;
-        st->ms_bin += T.ms(0, 1);
-        st->ms_voxel += T.ms(2, 3);          // the voxel kernel alone
-        st->ms_integrate += T.ms(4, 5);      // the integration kernels alone
-        st->ms_total += T.ms(0, 6);
-    }
-    if (plan_out) *plan_out = plan;
-    return D2BI_OK;
-}
-
-__global__ void k_div_f64(double* __restrict__ p, int64_t n, double d)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = p[i] / d;
-}
-
-// computeD2BI_img with apertureAngle and nAperturePoints > 1
-// (Lib_DynamicalTwoBeamImaging.f90:1775-1786, 1853-1866, 1905-1951).  Every
-// aperture point has its own g-vector for g.r (findGdotR_aperture :1269-1396)
-// and so its own phase field; the link cells, ranges and densities are those
-// of the unshifted condition.  Each point is binned, voxelised and integrated
-// at its nPrec deviation parameters aa(app, :), the column sums are added up
-// and the image is divided by nApp * nPrec at the end (the reference adds
-// over (prec, app) in the other nesting; FP64 sums agree to rounding).
-// Result left in ws.img.
-static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
-                          double apAngle, int nApp, int mode, d2bi_stats* st, d2bi_plan* plan_out)
-{
-    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
-    d2bi_plan plan;
-    CHK(plan_image(r, n, precAngle, nPrec, &plan));
-    std::vector<double> gshift(3 * (size_t)nApp), aa((size_t)nApp * nPrec);
-    CHK(aperture_setup(r, precAngle, nPrec, apAngle, nApp, gshift.data(), aa.data()));
-    AxisTables tab;
-    CHK(axis_tables(r, &plan, &tab));
-    hipStream_t s = g.stream;
-    AxisDev ax;
-    CHK(upload_axis(tab, &ax, s));
-    Workspace& w = g.ws;
-    const int64_t plane = (int64_t)plan.Mx * plan.My;
-    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
-    double* d_img = w.img.as<double>();
-    const bool dens = r->use_density != 0;
-    const double total = (double)nApp * nPrec;
-    for (int app = 0; app < nApp; app++) {
-        d2bi_plan pk = plan;
-        for (int i = 0; i < 3; i++) pk.gg[i] = gshift[3 * app + i];
-        for (int q = 0; q < nPrec; q++) pk.aa[q] = aa[app + (size_t)nApp * q];
-        BinParams BP;
-        CHK(bin_params(r, &pk, &BP));
-        BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
-        Binned b;
-        CHK(run_binning(d_x, n, BP, &b, s));
-        CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s));
-        if (mode == D2BI_MODE_FP64) {
-            double* d_aa = d_img + plane;
-            HIPCHK(hipMemcpyAsync(d_aa, pk.aa, sizeof(double) * nPrec, hipMemcpyHostToDevice, s));
-            // app 0 writes its sum (div 1), the others add theirs
-            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0));
-        } else {
-            CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total));
-        }
-        int64_t* hc = nullptr;
-        if (st) {
-            void* hp;
-            CHK(pinned(16, &hp));
-            hc = (int64_t*)hp;
-            HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
-        }
-        // pk.aa (host) is read by the copy above: finish before the next point
-        HIPCHK(hipStreamSynchronize(s));
-        if (st) {
-            st->n_points += b.nr;
-            st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
-            st->max_per_cell = std::max(st->max_per_cell, (int)(hc[0] & 0xffffffff));
-            st->n_cells = b.ncell;
-        }
-    }
-    if (mode == D2BI_MODE_FP64) {
-        hipLaunchKernelGGL(k_div_f64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, d_img, plane, total);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(s));
-    }
-    if (plan_out) *plan_out = plan;
-    return D2BI_OK;
-}
-
-static int upload_atoms_tmp(const double* x, int64_t ld, int64_t n, double** d_x)
-{
-    if (n < 0 || (n > 0 && (!x || ld < 3))) return seterr(D2BI_ERR_ARG, "bad atom array");
-    CHK(g.ws.x.ensure(sizeof(double) * 3 * std::max<int64_t>(n, 1)));
-    void* hp;
-    CHK(pinned(sizeof(double) * 3 * std::max<int64_t>(n, 1), &hp));
-    double* h = (double*)hp;
-    for (int64_t j = 0; j < n; j++) {
-        h[3 * j] = x[ld * j]; h[3 * j + 1] = x[ld * j + 1]; h[3 * j + 2] = x[ld * j + 2];
-    }
-    if (n > 0) HIPCHK(hipMemcpyAsync(g.ws.x.p, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    *d_x = g.ws.x.as<double>();
-    return D2BI_OK;
-}
-
-
-// ------------------------------------------------------------------ batched conditions
-
-__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-static void fill_i32(int* p, int64_t n, int v, hipStream_t s)
-{
-    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
-}
-
-// One condition of a batch: its plan, tables and slices of the batch buffers.
-struct Cond {
-    d2bi_plan plan;
-    AxisTables tab;
-    BinParams BP;
-    const double* d_x = nullptr;
-    int64_t n = 0;
-    int nblk = 0, ncell = 0, variant = 0;
-    bool dens = false;
-    int64_t nr = 0, nbc = 0, nvox = 0, plane = 0;
-    int64_t o_mask = 0, o_blk = 0, o_ax = 0, o_axs = 0, o_axq = 0, o_img = 0, o_aa = 0;
-    AxisDev ax;
-};
-
-struct Batch {
-    Buf mask, blkcnt, blkoff, axc, axz, axs, axq, img, aa, guard, mpc;
-    std::vector<unsigned char> key;
-    uint64_t gen = 0;
-    hipGraphExec_t exec = nullptr;
-    std::vector<int64_t> nr;
-    int64_t total_plane = 0;
-    void reset()
-    {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        exec = nullptr;
-        key.clear();
-    }
-    void release()
-    {
-        reset();
-        Buf* all[] = {&mask, &blkcnt, &blkoff, &axc, &axz, &axs, &img, &aa, &guard, &mpc};
-        for (Buf* b : all) b->release();
-    }
-};
-
-static Batch g_batch;
-
-static void key_append(std::vector<unsigned char>& k, const void* p, size_t n)
-{
-    const unsigned char* c = (const unsigned char*)p;
-    k.insert(k.end(), c, c + n);
-}
-
-// counting pass of one condition: replica predicate and per-block offsets
-static int enqueue_count(const Cond& c, Batch& B, int* scan_tmp, hipStream_t s)
-{
-    uint32_t* mask = B.mask.as<uint32_t>() + c.o_mask;
-    int* cnt = B.blkcnt.as<int>() + c.o_blk;
-    int* off = B.blkoff.as<int>() + c.o_blk;
-    if (c.n > 0)
-        hipLaunchKernelGGL(k_bin_classify, dim3(c.nblk), dim3(BIN_BLOCK), 0, s, c.d_x, c.BP, mask, cnt, c.nblk);
-    else
-        fill_i32(cnt, c.nbc, 0, s);
-    HIPCHK(hipGetLastError());
-    return scan_exclusive_tmp(cnt, off, c.nbc, scan_tmp, s);
-}
-
-// the rest of one condition: bin, voxelise, integrate into its image slice
-static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, int* scan_tmp, hipStream_t s)
-{
-    Workspace& w = g.ws;
-    const d2bi_plan* p = &c.plan;
-    int* mpc = B.mpc.as<int>() + idx;
-    unsigned long long* gc = B.guard.as<unsigned long long>() + idx;
-    fill_i32(w.cell_cnt.as<int>(), c.ncell + 1, 0, s);
-    fill_i32(mpc, 1, 0, s);
-    unsigned long long* gp = w.guard_part.as<unsigned long long>();
-    fill_i32((int*)gc, 2, 0, s);
-    if (mode == D2BI_MODE_FAST) fill_i32((int*)gp, 2 * GUARD_SLOTS * GUARD_STRIDE, 0, s);
-    if (c.n > 0 && c.nr > 0)
-        hipLaunchKernelGGL(k_bin_scatter, dim3(c.nblk), dim3(BIN_BLOCK), 0, s, c.d_x, c.BP,
-                           B.mask.as<uint32_t>() + c.o_mask, B.blkoff.as<int>() + c.o_blk, c.nblk,
-                           w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.pt_frac.as<double>(),
-                           w.pt_gdotr.as<double>(), w.pt_p32.as<float4>(), w.pt_sn32.as<float>(),
-                           w.cell_cnt.as<int>());
-    CHK(scan_exclusive_tmp(w.cell_cnt.as<int>(), w.cell_start.as<int>(), c.ncell, scan_tmp, s));
-    if (c.nr > 0)
-        hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((c.nr + 255) / 256)), dim3(256), 0, s, c.nr,
-                           w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.cell_start.as<int>(), w.cell_ids.as<int>());
-    launch_bin_finish(c.ncell, c.nr, c.BP, mpc, s);
-    HIPCHK(hipGetLastError());
-    VoxParams V;
-    d2bi_record dummy;
-    dummy.use_density = c.dens ? 1 : 0;
-    fill_vox_params(&dummy, p, c.ax, &V);
-    double* d_img = B.img.as<double>() + c.o_img;
-    if (mode == D2BI_MODE_FP64) {
-        dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
-        hipLaunchKernelGGL(k_voxel_fp64, grid, dim3(256), 0, s, V, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
-                           w.slot_cs64.as<double2>(), w.f64_re.as<double>(), w.f64_im.as<double>(),
-                           w.f64_rho.as<double>(), (int*)nullptr, (double*)nullptr);
-        HIPCHK(hipGetLastError());
-        return launch_integrate_fp64(p, B.aa.as<double>() + c.o_aa, d_img, s);
-    }
-    FastParams F;
-    int variant = c.variant;
-    CHK(variant_params(p, c.tab, V, &variant, &F));
-    launch_tile<false>(F, variant, c.dens, w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
-                       w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
-                       gp, s);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
-    return launch_integrate_fast(p, c.dens, w.f32_field.p, d_img, s);
-}
-
-// Batched conditions.  The first call with a given set of inputs plans every
-// condition, counts the points of each (one host synchronisation for the
-// sizes), sizes the workspace for the largest, and captures the whole batch
-// (counting, binning, voxelisation, integration of every condition) into one
-// HIP graph.  Later calls with the same inputs and an unchanged workspace
-// replay the graph: every replay recomputes everything from the atoms.
-// img_out: host memory, or device memory when dev_out.
-static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec, const double* precAngle,
-                     const int32_t* nPrec, int mode, double* img_out, bool dev_out, d2bi_stats* st)
-{
-    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
-    Batch& B = g_batch;
-    hipStream_t s = g.stream;
-    std::vector<unsigned char> key;
-    key_append(key, &mode, sizeof mode);
-    key_append(key, &nRec, sizeof nRec);
-    const int fv = forced_variant();
-    key_append(key, &fv, sizeof fv);
-    for (int i = 0; i < nRec; i++) {
-        key_append(key, &recs[i], sizeof(d2bi_record));
-        key_append(key, &atoms[i]->x, sizeof(double*));
-        key_append(key, &atoms[i]->n, sizeof(int64_t));
-        key_append(key, &precAngle[i], sizeof(double));
-        key_append(key, &nPrec[i], sizeof(int32_t));
-    }
-    const bool replay = B.exec && key == B.key && B.gen == g_alloc_gen;
-    if (!replay) {
-        B.reset();
-        std::vector<Cond> C(nRec);
-        int64_t o_mask = 0, o_blk = 0, o_ax = 0, o_axs = 0, o_axq = 0, o_img = 0, o_aa = 0;
-        int64_t max_scan = 1;
-        for (int i = 0; i < nRec; i++) {
-            Cond& c = C[i];
-            CHK(plan_image(&recs[i], atoms[i]->n, precAngle[i], nPrec[i], &c.plan));
-            CHK(axis_tables(&recs[i], &c.plan, &c.tab));
-            CHK(bin_params(&recs[i], &c.plan, &c.BP));
-            c.BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
-            c.d_x = atoms[i]->x;
-            c.n = atoms[i]->n;
-            c.BP.n = c.n;
-            c.nblk = (int)std::max<int64_t>(1, (c.n + BIN_TILE - 1) / BIN_TILE);
-            c.nbc = (int64_t)c.BP.nuvw * c.nblk;
-            c.ncell = c.plan.Ex * c.plan.Ey * c.plan.Ez;
-            c.dens = recs[i].use_density != 0;
-            c.plane = (int64_t)c.plan.Mx * c.plan.My;
-            c.nvox = c.plane * c.plan.Mz;
-            c.o_mask = o_mask; o_mask += std::max<int64_t>(c.n, 1);
-            c.o_blk = o_blk; o_blk += c.nbc + 1;
-            c.o_ax = o_ax; o_ax += c.plan.Mx + c.plan.My + c.plan.Mz;
-            c.o_axq = o_axq; o_axq += c.plan.Mx + c.plan.My + 2 * (int64_t)c.plan.Mz;
-            c.o_axs = o_axs; o_axs += c.plan.Ex + c.plan.Ey + c.plan.Ez + 3;
-            c.o_img = o_img; o_img += c.plane;
-            c.o_aa = o_aa; o_aa += c.plan.nPrec;
-            max_scan = std::max(max_scan, std::max(scan_tmp_ints(c.nbc), scan_tmp_ints(c.ncell)));
-        }
-        CHK(B.mask.ensure(sizeof(uint32_t) * o_mask));
-        CHK(B.blkcnt.ensure(sizeof(int) * o_blk));
-        CHK(B.blkoff.ensure(sizeof(int) * o_blk));
-        CHK(B.axc.ensure(sizeof(int) * o_ax));
-        CHK(B.axz.ensure(sizeof(double) * o_ax));
-        CHK(B.axs.ensure(sizeof(int) * o_axs));
-        CHK(B.axq.ensure(sizeof(QL) * o_axq));
-        CHK(B.img.ensure(sizeof(double) * o_img));
-        CHK(B.aa.ensure(sizeof(double) * std::max<int64_t>(o_aa, 1)));
-        CHK(B.guard.ensure(sizeof(unsigned long long) * nRec));
-        CHK(g.ws.guard_part.ensure(sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE));
-        CHK(B.mpc.ensure(sizeof(int) * nRec));
-        CHK(g.ws.scan_tmp.ensure(sizeof(int) * max_scan));
-        // tables and precession coefficients stay resident in the batch buffers
-        {
-            std::vector<int> hc(o_ax), hs(o_axs);
-            std::vector<double> hz(o_ax), haa(std::max<int64_t>(o_aa, 1));
-            std::vector<QL> hq(o_axq);
-            for (int i = 0; i < nRec; i++) {
-                Cond& c = C[i];
-                int64_t oc = c.o_ax, os = c.o_axs, oq = c.o_axq;
-                for (int a = 0; a < 4; a++) {
-                    std::copy(c.tab.ql[a].begin(), c.tab.ql[a].end(), hq.begin() + oq);
-                    c.ax.ql[a] = B.axq.as<QL>() + oq;
-                    oq += (int64_t)c.tab.ql[a].size();
-                }
-                for (int a = 0; a < 3; a++) {
-                    std::copy(c.tab.cell[a].begin(), c.tab.cell[a].end(), hc.begin() + oc);
-                    std::copy(c.tab.zi[a].begin(), c.tab.zi[a].end(), hz.begin() + oc);
-                    std::copy(c.tab.start[a].begin(), c.tab.start[a].end(), hs.begin() + os);
-                    c.ax.cell[a] = B.axc.as<int>() + oc;
-                    c.ax.zi[a] = B.axz.as<double>() + oc;
-                    c.ax.start[a] = B.axs.as<int>() + os;
-                    oc += (int64_t)c.tab.cell[a].size();
-                    os += (int64_t)c.tab.start[a].size();
-                }
-                std::copy(c.plan.aa, c.plan.aa + c.plan.nPrec, haa.begin() + c.o_aa);
-            }
-            HIPCHK(hipMemcpy(B.axc.p, hc.data(), sizeof(int) * o_ax, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.axz.p, hz.data(), sizeof(double) * o_ax, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.axs.p, hs.data(), sizeof(int) * o_axs, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.axq.p, hq.data(), sizeof(QL) * o_axq, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.aa.p, haa.data(), sizeof(double) * haa.size(), hipMemcpyHostToDevice));
-        }
-        // counting pass: sizes of every condition
-        int* scan_tmp = g.ws.scan_tmp.as<int>();
-        for (int i = 0; i < nRec; i++) CHK(enqueue_count(C[i], B, scan_tmp, s));
-        std::vector<int> tot(nRec);
-        for (int i = 0; i < nRec; i++)
-            HIPCHK(hipMemcpyAsync(&tot[i], B.blkoff.as<int>() + C[i].o_blk + C[i].nbc, sizeof(int),
-                                  hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        int64_t max_nr = 1, max_cell = 1, max_vox = 1;
-        for (int i = 0; i < nRec; i++) {
-            if (tot[i] < 0) return seterr(D2BI_ERR_CAPACITY, "point count overflow");
-            C[i].nr = tot[i];
-            max_nr = std::max(max_nr, C[i].nr);
-            max_cell = std::max<int64_t>(max_cell, C[i].ncell + 1);
-            max_vox = std::max(max_vox, C[i].nvox);
-        }
-        Workspace& w = g.ws;
-        CHK(w.pt_cell.ensure(sizeof(int) * max_nr));
-        CHK(w.pt_slot.ensure(sizeof(int) * max_nr));
-        CHK(w.pt_frac.ensure(sizeof(double) * 3 * max_nr));
-        CHK(w.pt_gdotr.ensure(sizeof(double) * max_nr));
-        CHK(w.pt_p32.ensure(sizeof(float4) * max_nr));
-        CHK(w.pt_sn32.ensure(sizeof(float) * max_nr));
-        CHK(w.cell_ids.ensure(sizeof(int) * max_nr));
-        CHK(w.slot_id.ensure(sizeof(int) * max_nr));
-        CHK(w.slot_frac.ensure(sizeof(double4) * max_nr));
-        CHK(w.slot_cs64.ensure(sizeof(double2) * max_nr));
-        CHK(w.slot_p32.ensure(sizeof(float4) * max_nr));
-        CHK(w.slot_sn32.ensure(sizeof(float) * max_nr));
-        CHK(w.cell_cnt.ensure(sizeof(int) * max_cell));
-        CHK(w.cell_start.ensure(sizeof(int) * max_cell));
-        CHK(w.blk_max.ensure(sizeof(int) * (max_cell / 256 + 2)));
-        if (mode == D2BI_MODE_FP64) {
-            CHK(w.f64_re.ensure(sizeof(double) * max_vox));
-            CHK(w.f64_im.ensure(sizeof(double) * max_vox));
-            CHK(w.f64_rho.ensure(sizeof(double) * max_vox));
-        } else {
-            CHK(w.f32_field.ensure(sizeof(float4) * max_vox));
-        }
-        // FAST kernel variants: tune unseen shapes eagerly (outside the graph)
-        for (int i = 0; i < nRec && mode == D2BI_MODE_FAST; i++) {
-            int v = known_variant(&C[i].plan, C[i].dens);
-            if (v < 0) {
-                d2bi_plan tmp;
-                CHK(image_condition(&recs[i], C[i].d_x, C[i].n, precAngle[i], nPrec[i], mode, nullptr, &tmp));
-                v = known_variant(&C[i].plan, C[i].dens);
-                if (v < 0) v = 0;
-            }
-            C[i].variant = v;
-        }
-        // the eager tuning may have grown workspace buffers: capture after it
-        hipGraph_t graph;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = D2BI_OK;
-        for (int i = 0; i < nRec && rc == D2BI_OK; i++) {
-            rc = enqueue_count(C[i], B, g.ws.scan_tmp.as<int>(), s);
-            if (rc == D2BI_OK) rc = enqueue_rest(C[i], i, B, mode, g.ws.scan_tmp.as<int>(), s);
-        }
-        hipError_t e = hipStreamEndCapture(s, &graph);
-        if (rc != D2BI_OK) {
-            if (e == hipSuccess) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        if (e != hipSuccess) return seterr(D2BI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            B.exec = nullptr;
-            return seterr(D2BI_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
-        }
-        B.key = key;
-        B.gen = g_alloc_gen;
-        B.nr.resize(nRec);
-        for (int i = 0; i < nRec; i++) B.nr[i] = C[i].nr;
-        B.total_plane = o_img;
-    }
-    HIPCHK(hipEventRecord(g.ev[0], s));
-    HIPCHK(hipGraphLaunch(B.exec, s));
-    HIPCHK(hipEventRecord(g.ev[1], s));
-    HIPCHK(hipMemcpyAsync(img_out, B.img.p, sizeof(double) * B.total_plane,
-                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (st) {
-        std::vector<unsigned long long> gcv(nRec);
-        std::vector<int> mpc(nRec);
-        HIPCHK(hipMemcpy(gcv.data(), B.guard.p, sizeof(unsigned long long) * nRec, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(mpc.data(), B.mpc.p, sizeof(int) * nRec, hipMemcpyDeviceToHost));
-        for (int i = 0; i < nRec; i++) {
-            st->n_points += B.nr[i];
-            st->n_guard += (int64_t)gcv[i];
-            st->max_per_cell = std::max(st->max_per_cell, mpc[i]);
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[1]);
-        st->ms_total += ms;
-    }
-    return D2BI_OK;
-}
-
-}  // namespace d2bi
+// The parts, in order (one translation unit):
+#include "d2bi_common.cuh"
+#include "d2bi_bin.cuh"
+#include "d2bi_axis.cuh"
+#include "d2bi_fp64.cuh"
+#include "d2bi_voxel_tile.cuh"
+#include "d2bi_integrate_fast.cuh"
+#include "d2bi_blur_query.cuh"
+#include "d2bi_run.cuh"
+#include "d2bi_batch.cuh"
 
 // ------------------------------------------------------------------ C-ABI
 

@@ -1,0 +1,505 @@
+// d2bi_run.cuh -- one imaging condition: planning, binning, voxelisation (with the
+// FAST variant choice), integration, the aperture sum and the field API.
+//
+// One part of the single translation unit d2bi_gpu.hip, which includes the
+// parts in order so that kernels and the device helpers they call compile
+// together; each part uses the definitions of the parts before it.
+#pragma once
+
+namespace d2bi {
+
+// ------------------------------------------------------------------ orchestration
+
+struct Timer {
+    hipEvent_t* ev;
+    hipStream_t s;
+    int record(int i)
+    {
+        HIPCHK(hipEventRecord(ev[i], s));
+        return D2BI_OK;
+    }
+    double ms(int a, int b)
+    {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+        return t;
+    }
+};
+
+static void fill_vox_params(const d2bi_record* r, const d2bi_plan* p, const AxisDev& ax, VoxParams* V)
+{
+    V->Mx = p->Mx; V->My = p->My; V->Mz = p->Mz;
+    V->Ex = p->Ex; V->Ey = p->Ey; V->Ez = p->Ez;
+    V->a[0] = p->a_col[0]; V->a[1] = p->a_col[4]; V->a[2] = p->a_col[8];
+    V->rc2 = p->rc2; V->i2s2 = p->i2s2; V->sigma = p->sigma;
+    V->use_density = r->use_density ? 1 : 0;
+    V->ax = ax;
+}
+
+// Tile depth of a FAST kernel variant: 0 (a lane per voxel) and 1 (rows of
+// 4 x 4 x 4 tiles) use 4 voxels, 2 (rows of 4 x 4 x 8 tiles) 8.
+static int variant_tz(int variant) { return variant == 2 ? 8 : 4; }
+
+// Link cells in the largest unwrapped cell range of a 4 x 4 x tz tile.
+static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz)
+{
+    int worst = 1;
+    for (int a = 0; a < 3; a++) {
+        const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
+        const int ts = a == 2 ? tz : TILE;
+        int mx = 0;
+        for (int t0 = 0; t0 < M; t0 += ts)
+            mx = std::max(mx, tab.cell[a][std::min(M, t0 + ts) - 1] - tab.cell[a][t0] + 3);
+        worst *= mx;
+    }
+    return worst;
+}
+
+// Launch parameters of the FAST voxel kernel for a plan, tiles of 4 x 4 x tz.
+static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo, int tz)
+{
+    FastParams& F = *Fo;
+    F.v = V;
+    F.tz = tz;
+    F.ntx = (p->Mx + TILE - 1) / TILE;
+    F.nty = (p->My + TILE - 1) / TILE;
+    F.ntz = (p->Mz + tz - 1) / tz;
+    F.ntiles = F.ntx * F.nty * F.ntz;
+    F.per_xcd = (F.ntiles + 7) / 8;
+    // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
+    // rounded once from FP64, three FMAs); +-1e-5 relative covers it 5x
+    const double k2 = -p->i2s2 * 1.4426950408889634;
+    F.k2 = (float)k2;
+    F.e_rc = (float)(k2 * p->rc2);
+    F.e_band = (float)(std::fabs(k2 * p->rc2) * 1.0e-5);
+    F.w_scale = (float)std::exp2((double)F.e_rc);
+    F.m2_min = (float)(1.0e-32 / ((double)F.w_scale * (double)F.w_scale));
+    F.inv_k2 = (float)(1.0 / k2);
+    F.rc_cull2 = (float)(p->rc2 * (1.0 + 1.0e-4));
+    F.cellf[0] = (float)p->a_col[0]; F.cellf[1] = (float)p->a_col[4]; F.cellf[2] = (float)p->a_col[8];
+    // cells thinner than 3 sigma by more than rounding (the guard band, ~2e-5
+    // A, covers less): only z can be, x and y are wide by construction
+    const double rc = std::sqrt(p->rc2), thin = rc * (1.0 - 1.0e-7);
+    if (p->a_col[0] < thin || p->a_col[4] < thin)
+        return seterr(D2BI_ERR_ARG, "link cells %.6f x %.6f narrower than 3 sigma = %.6f", p->a_col[0],
+                      p->a_col[4], rc);
+    F.thin_z = p->a_col[8] < thin ? 1 : 0;
+    // the unwrapped cell range of a tile must fit the per-tile cell table
+    const int worst = tile_cells(p, tab, tz);
+    if (worst > TILE_MAXC) return seterr(D2BI_ERR_CAPACITY, "voxel tile spans %d link cells (max %d)", worst, TILE_MAXC);
+    return D2BI_OK;
+}
+
+// Launch parameters of a variant.  A 4 x 4 x 8 tile that spans more link
+// cells than the per-tile table holds falls back to the 4 x 4 x 4 row kernel
+// (*variant becomes 1).
+static int variant_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, int* variant,
+                          FastParams* F)
+{
+    if (*variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) *variant = 1;
+    return fast_params(p, tab, V, F, variant_tz(*variant));
+}
+
+// Kernel instance for (diagnostics, density, variant); link cells thinner
+// than 3 sigma along z take a row kernel (variant 1 or 2) with the z-offset
+// test.
+template <bool DENS, int ROW, class Go>
+static void pick_row(int tz, Go go)
+{
+    if (tz == 8) go(k_voxel_tile<DENS, false, ROW, 8>);
+    else go(k_voxel_tile<DENS, false, ROW, 4>);
+}
+
+template <class Go>
+static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
+{
+    if (diag) { go(k_voxel_tile<true, true, 0, 4>); return; }
+    const int tz = variant_tz(variant);
+    if (!thin_z && variant == 0) {
+        if (dens) go(k_voxel_tile<true, false, 0, 4>);
+        else go(k_voxel_tile<false, false, 0, 4>);
+    } else if (thin_z) {
+        if (dens) pick_row<true, 2>(tz, go);
+        else pick_row<false, 2>(tz, go);
+    } else {
+        if (dens) pick_row<true, 1>(tz, go);
+        else pick_row<false, 1>(tz, go);
+    }
+}
+
+template <bool DIAG>
+static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start, const int* cell_ids,
+                        const double* pt_frac, const float4* p32, const float* sn32, float4* field, int* cnt,
+                        float* r2m, unsigned long long* gc, hipStream_t s)
+{
+    pick_tile(DIAG, dens, variant, F.thin_z != 0, [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, cell_ids, pt_frac, p32, sn32,
+                           field, cnt, r2m, gc);
+    });
+}
+
+// Choice of the FAST voxel kernel variant (0: a lane per voxel, 1: a lane
+// per row of four voxels and quarter of the candidates of a 4 x 4 x 4 tile,
+// 2: a lane per row and half of the candidates of a 4 x 4 x 8 tile) per
+// voxel-grid shape: timed once on the first condition of that shape, then
+// cached.  D2BI_FAST_VARIANT=0|1|2 forces one.
+static std::map<std::array<int, 7>, int> g_variant_cache;
+
+static int forced_variant()
+{
+    const char* env = std::getenv("D2BI_FAST_VARIANT");
+    if (env && env[0] >= '0' && env[0] <= '2' && env[1] == 0) return env[0] - '0';
+    return -1;
+}
+
+// cached (or forced) variant for a shape, -1 if not tuned yet
+static int known_variant(const d2bi_plan* p, bool dens)
+{
+    const int f = forced_variant();
+    if (f >= 0) return f;
+    const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
+    auto it = g_variant_cache.find(key);
+    return it == g_variant_cache.end() ? -1 : it->second;
+}
+
+template <class TimeFn>
+static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, TimeFn time_variant)
+{
+    auto& cache = g_variant_cache;
+    const int f = forced_variant();
+    if (f >= 0) return f;
+    if (diag) return 0;
+    const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    // each variant twice, the second (warm-cache) timing counts; thin z
+    // cells have only the row kernels
+    const int v0 = thin_z ? 1 : 0;
+    float t[3] = {0.0f, 0.0f, 0.0f};
+    for (int rep = 0; rep < 2; rep++)
+        for (int v = v0; v < 3; v++) {
+            t[v] = time_variant(v);
+            if (t[v] < 0) return seterr(D2BI_ERR_HIP, "variant timing failed");
+        }
+    int best = v0;
+    for (int v = v0 + 1; v < 3; v++)
+        if (t[v] < t[best]) best = v;
+    cache[key] = best;
+    return best;
+}
+
+// Voxelise into the mode's field buffers; optional count / r2min outputs.
+static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables& tab, const AxisDev& ax,
+                     int mode, bool want_diag, int64_t* n_guard, hipStream_t s, Timer* T = nullptr)
+{
+    Workspace& w = g.ws;
+    const int64_t nvox = (int64_t)p->Mx * p->My * p->Mz;
+    VoxParams V;
+    fill_vox_params(r, p, ax, &V);
+    int* dcnt = nullptr;
+    void* dr2 = nullptr;
+    if (want_diag) {
+        CHK(w.cnt.ensure(sizeof(int) * nvox));
+        CHK(w.r2m.ensure(sizeof(double) * nvox));
+        dcnt = w.cnt.as<int>();
+        dr2 = w.r2m.p;
+    }
+    if (mode == D2BI_MODE_FP64) {
+        CHK(w.f64_re.ensure(sizeof(double) * nvox));
+        CHK(w.f64_im.ensure(sizeof(double) * nvox));
+        CHK(w.f64_rho.ensure(sizeof(double) * nvox));
+        dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
+        if (T) CHK(T->record(2));
+        hipLaunchKernelGGL(k_voxel_fp64, grid, dim3(256), 0, s, V, w.cell_start.as<int>(),
+                           w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), w.f64_re.as<double>(),
+                           w.f64_im.as<double>(), w.f64_rho.as<double>(), dcnt, (double*)dr2);
+        HIPCHK(hipGetLastError());
+        if (T) CHK(T->record(3));
+        return D2BI_OK;
+    }
+    CHK(w.f32_field.ensure(sizeof(float4) * nvox));
+    FastParams F;
+    CHK(fast_params(p, tab, V, &F, 4));
+    CHK(w.counters.ensure(64));
+    unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
+    const size_t gp_bytes = sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE;
+    CHK(w.guard_part.ensure(gp_bytes));
+    unsigned long long* gp = w.guard_part.as<unsigned long long>();
+    const bool dens = r->use_density != 0;
+    // launch parameters follow the variant's tile depth
+    auto run_variant = [&](int v) -> int {
+        FastParams Fv;
+        CHK(variant_params(p, tab, V, &v, &Fv));
+        pick_tile(want_diag, dens, v, Fv.thin_z != 0, [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(8 * Fv.per_xcd), dim3(64), 0, s, Fv, w.cell_start.as<int>(),
+                               w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
+                               w.slot_sn32.as<float>(), w.f32_field.as<float4>(), dcnt, (float*)dr2, gp);
+        });
+        return D2BI_OK;
+    };
+    const int variant = fast_variant(p, dens, want_diag, F.thin_z != 0, [&](int v) -> float {
+        // a variant whose tiles span too many link cells is not timed
+        if (tile_cells(p, tab, variant_tz(v)) > TILE_MAXC) return 3.4e38f;
+        HIPCHK(hipEventRecord(g.ev[7], s));
+        CHK(run_variant(v));
+        HIPCHK(hipEventRecord(g.ev[8], s));
+        HIPCHK(hipEventSynchronize(g.ev[8]));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, g.ev[7], g.ev[8]);
+        return ms;
+    });
+    if (variant < 0) return variant;
+    HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches counted too
+    if (T) CHK(T->record(2));
+    CHK(run_variant(variant));
+    HIPCHK(hipGetLastError());
+    if (T) CHK(T->record(3));
+    hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
+    if (n_guard) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, gc, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *n_guard = (int64_t)v;
+    }
+    return D2BI_OK;
+}
+
+// FAST integration of 8 or more precession points per pass through the
+// shared step matrix (k_integrate_poly); D2BI_FAST_INTEGRATOR=stage keeps the
+// stage-by-stage march for every pass.
+static bool fast_integrator_poly()
+{
+    const char* e = std::getenv("D2BI_FAST_INTEGRATOR");
+    return !(e && std::strcmp(e, "stage") == 0);
+}
+
+template <int NP>
+static void launch_int_fast(const FastIntParams& P, bool dens, const void* field, double* img, int acc,
+                            double inv, hipStream_t s)
+{
+    const int64_t plane = (int64_t)P.Mx * P.My;
+    const dim3 grid((unsigned)((plane + 255) / 256));
+    // from 8 points on, the shared step matrix costs less than it saves
+    if constexpr (NP >= 8) {
+        if (fast_integrator_poly()) {
+            if (dens) hipLaunchKernelGGL((k_integrate_poly<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+            else hipLaunchKernelGGL((k_integrate_poly<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+            return;
+        }
+    }
+    if (dens) hipLaunchKernelGGL((k_integrate_fast<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+    else hipLaunchKernelGGL((k_integrate_fast<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+}
+
+// FAST integration of all precession points, in passes of up to 32.
+// dens: the field holds float4 {x, rho} (density field on), else float2 x.
+// acc_in: add to d_img from the first pass on; inv: weight of one
+// integration (0: 1/nPrec).
+static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, double* d_img, hipStream_t s,
+                                 bool acc_in = false, double inv = 0.0)
+{
+    if (inv == 0.0) inv = 1.0 / p->nPrec;
+    for (int p0 = 0, first = acc_in ? 0 : 1; p0 < p->nPrec; first = 0) {
+        const int left = p->nPrec - p0;
+        const int np = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+        FastIntParams P;
+        P.Mx = p->Mx; P.My = p->My; P.Mz = p->Mz; P.np = np;
+        P.b0 = (float)p->b0; P.bg = (float)p->bg; P.dz = (float)p->dz;
+        for (int i = 0; i < 32; i++) {
+            P.aa[i] = i < np ? (float)p->aa[p0 + i] : 0.0f;
+            P.al[i] = i < np ? (float)(2.0 * p->dz * p->aa[p0 + i]) : 0.0f;
+        }
+        switch (np) {
+            case 32: launch_int_fast<32>(P, dens, f, d_img, !first, inv, s); break;
+            case 16: launch_int_fast<16>(P, dens, f, d_img, !first, inv, s); break;
+            case 8: launch_int_fast<8>(P, dens, f, d_img, !first, inv, s); break;
+            case 4: launch_int_fast<4>(P, dens, f, d_img, !first, inv, s); break;
+            case 2: launch_int_fast<2>(P, dens, f, d_img, !first, inv, s); break;
+            default: launch_int_fast<1>(P, dens, f, d_img, !first, inv, s); break;
+        }
+        HIPCHK(hipGetLastError());
+        p0 += np;
+    }
+    return D2BI_OK;
+}
+
+// accumulate: add each column's sum over precession points to d_img;
+// otherwise write sum / nPrec (div = 0) or sum / div.
+static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double* d_img, hipStream_t s,
+                                 int accumulate = 0, double div = 0.0)
+{
+    Workspace& w = g.ws;
+    const int64_t plane = (int64_t)p->Mx * p->My;
+    IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa, accumulate,
+                div == 0.0 ? (double)p->nPrec : div};
+    hipLaunchKernelGGL(k_integrate_fp64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
+                       w.f64_re.as<double>(), w.f64_im.as<double>(), w.f64_rho.as<double>(), d_img);
+    HIPCHK(hipGetLastError());
+    return D2BI_OK;
+}
+
+static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s, Timer* T = nullptr)
+{
+    Workspace& w = g.ws;
+    const int64_t plane = (int64_t)p->Mx * p->My;
+    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
+    double* d_img = w.img.as<double>();
+    if (mode == D2BI_MODE_FP64) {
+        double* d_aa = d_img + plane;
+        HIPCHK(hipMemcpyAsync(d_aa, p->aa, sizeof(double) * p->nPrec, hipMemcpyHostToDevice, s));
+        if (T) CHK(T->record(4));
+        CHK(launch_integrate_fp64(p, d_aa, d_img, s));
+        if (T) CHK(T->record(5));
+        // the aa copy reads p->aa (caller memory): finish before returning
+        HIPCHK(hipStreamSynchronize(s));
+        return D2BI_OK;
+    }
+    if (T) CHK(T->record(4));
+    CHK(launch_integrate_fast(p, dens, w.f32_field.p, d_img, s));
+    if (T) CHK(T->record(5));
+    return D2BI_OK;
+}
+
+// Whole condition: bin -> voxelise -> integrate, result left in ws.img.
+static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
+                           int mode, d2bi_stats* st, d2bi_plan* plan_out)
+{
+    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
+    d2bi_plan plan;
+    CHK(plan_image(r, n, precAngle, nPrec, &plan));
+    AxisTables tab;
+    CHK(axis_tables(r, &plan, &tab));
+    BinParams BP;
+    CHK(bin_params(r, &plan, &BP));
+    BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
+    hipStream_t s = g.stream;
+    Timer T{g.ev, s};
+    CHK(T.record(0));
+    AxisDev ax;
+    CHK(upload_axis(tab, &ax, s));
+    Binned b;
+    CHK(run_binning(d_x, n, BP, &b, s));
+    CHK(T.record(1));
+    CHK(run_voxel(r, &plan, tab, ax, mode, false, nullptr, s, &T));
+    CHK(run_integrate(&plan, r->use_density != 0, mode, s, &T));
+    CHK(T.record(6));
+    // counters (largest cell count at 0, FAST guard re-tests at 8) come back
+    // with the final synchronisation
+    int64_t* hc = nullptr;
+    if (st) {
+        void* hp;
+        CHK(pinned(16, &hp));
+        hc = (int64_t*)hp;
+        HIPCHK(hipMemcpyAsync(hc, g.ws.counters.p, 16, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        const int mpc = (int)(hc[0] & 0xffffffff);
+        st->n_points += b.nr;
+        st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
+        st->max_per_cell = std::max(st->max_per_cell, mpc);
+        st->n_cells = b.ncell;
+        st->ms_bin += T.ms(0, 1);
+        st->ms_voxel += T.ms(2, 3);          // the voxel kernel alone
+        st->ms_integrate += T.ms(4, 5);      // the integration kernels alone
+        st->ms_total += T.ms(0, 6);
+    }
+    if (plan_out) *plan_out = plan;
+    return D2BI_OK;
+}
+
+__global__ void k_div_f64(double* __restrict__ p, int64_t n, double d)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = p[i] / d;
+}
+
+// computeD2BI_img with apertureAngle and nAperturePoints > 1
+// (Lib_DynamicalTwoBeamImaging.f90:1775-1786, 1853-1866, 1905-1951).  Every
+// aperture point has its own g-vector for g.r (findGdotR_aperture :1269-1396)
+// and so its own phase field; the link cells, ranges and densities are those
+// of the unshifted condition.  Each point is binned, voxelised and integrated
+// at its nPrec deviation parameters aa(app, :), the column sums are added up
+// and the image is divided by nApp * nPrec at the end (the reference adds
+// over (prec, app) in the other nesting; FP64 sums agree to rounding).
+// Result left in ws.img.
+static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
+                          double apAngle, int nApp, int mode, d2bi_stats* st, d2bi_plan* plan_out)
+{
+    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
+    d2bi_plan plan;
+    CHK(plan_image(r, n, precAngle, nPrec, &plan));
+    std::vector<double> gshift(3 * (size_t)nApp), aa((size_t)nApp * nPrec);
+    CHK(aperture_setup(r, precAngle, nPrec, apAngle, nApp, gshift.data(), aa.data()));
+    AxisTables tab;
+    CHK(axis_tables(r, &plan, &tab));
+    hipStream_t s = g.stream;
+    AxisDev ax;
+    CHK(upload_axis(tab, &ax, s));
+    Workspace& w = g.ws;
+    const int64_t plane = (int64_t)plan.Mx * plan.My;
+    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
+    double* d_img = w.img.as<double>();
+    const bool dens = r->use_density != 0;
+    const double total = (double)nApp * nPrec;
+    for (int app = 0; app < nApp; app++) {
+        d2bi_plan pk = plan;
+        for (int i = 0; i < 3; i++) pk.gg[i] = gshift[3 * app + i];
+        for (int q = 0; q < nPrec; q++) pk.aa[q] = aa[app + (size_t)nApp * q];
+        BinParams BP;
+        CHK(bin_params(r, &pk, &BP));
+        BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
+        Binned b;
+        CHK(run_binning(d_x, n, BP, &b, s));
+        CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s));
+        if (mode == D2BI_MODE_FP64) {
+            double* d_aa = d_img + plane;
+            HIPCHK(hipMemcpyAsync(d_aa, pk.aa, sizeof(double) * nPrec, hipMemcpyHostToDevice, s));
+            // app 0 writes its sum (div 1), the others add theirs
+            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0));
+        } else {
+            CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total));
+        }
+        int64_t* hc = nullptr;
+        if (st) {
+            void* hp;
+            CHK(pinned(16, &hp));
+            hc = (int64_t*)hp;
+            HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
+        }
+        // pk.aa (host) is read by the copy above: finish before the next point
+        HIPCHK(hipStreamSynchronize(s));
+        if (st) {
+            st->n_points += b.nr;
+            st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
+            st->max_per_cell = std::max(st->max_per_cell, (int)(hc[0] & 0xffffffff));
+            st->n_cells = b.ncell;
+        }
+    }
+    if (mode == D2BI_MODE_FP64) {
+        hipLaunchKernelGGL(k_div_f64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, d_img, plane, total);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (plan_out) *plan_out = plan;
+    return D2BI_OK;
+}
+
+static int upload_atoms_tmp(const double* x, int64_t ld, int64_t n, double** d_x)
+{
+    if (n < 0 || (n > 0 && (!x || ld < 3))) return seterr(D2BI_ERR_ARG, "bad atom array");
+    CHK(g.ws.x.ensure(sizeof(double) * 3 * std::max<int64_t>(n, 1)));
+    void* hp;
+    CHK(pinned(sizeof(double) * 3 * std::max<int64_t>(n, 1), &hp));
+    double* h = (double*)hp;
+    for (int64_t j = 0; j < n; j++) {
+        h[3 * j] = x[ld * j]; h[3 * j + 1] = x[ld * j + 1]; h[3 * j + 2] = x[ld * j + 2];
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(g.ws.x.p, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    *d_x = g.ws.x.as<double>();
+    return D2BI_OK;
+}
+
+
+}  // namespace d2bi
