@@ -140,10 +140,21 @@ static void launch_tile(const FastParams& F, int variant, bool dens, const int* 
 
 // Choice of the FAST voxel kernel variant (0: a lane per voxel, 1: a lane
 // per row of four voxels and quarter of the candidates of a 4 x 4 x 4 tile,
-// 2: a lane per row and half of the candidates of a 4 x 4 x 8 tile) per
-// voxel-grid shape: timed once on the first condition of that shape, then
-// cached.  D2BI_FAST_VARIANT=0|1|2 forces one.
+// 2: a lane per row and half of the candidates of a 4 x 4 x 8 tile).  The
+// variants sum the same atoms in different orders, so their fields agree to
+// FP32 rounding, not bit for bit; the choice is therefore fixed, variant 2
+// (the fastest on every measured workload; variant_params falls back to 1 for
+// a tile whose cell range does not fit), so that a condition images the same
+// in every process.  D2BI_FAST_VARIANT=0|1|2 forces a variant;
+// D2BI_FAST_TUNE=1 instead times all three on the first condition of each
+// voxel-grid shape and keeps the fastest (results then depend on the timing).
 static std::map<std::array<int, 7>, int> g_variant_cache;
+
+static bool tune_variants()
+{
+    const char* e = std::getenv("D2BI_FAST_TUNE");
+    return e && e[0] == '1' && e[1] == 0;
+}
 
 static int forced_variant()
 {
@@ -152,11 +163,12 @@ static int forced_variant()
     return -1;
 }
 
-// cached (or forced) variant for a shape, -1 if not tuned yet
+// the variant for a shape: forced, fixed, or tuned (-1 if not tuned yet)
 static int known_variant(const d2bi_plan* p, bool dens)
 {
     const int f = forced_variant();
     if (f >= 0) return f;
+    if (!tune_variants()) return 2;
     const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
     auto it = g_variant_cache.find(key);
     return it == g_variant_cache.end() ? -1 : it->second;
@@ -169,6 +181,7 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, T
     const int f = forced_variant();
     if (f >= 0) return f;
     if (diag) return 0;
+    if (!tune_variants()) return 2;
     const std::array<int, 7> key{p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
