@@ -453,3 +453,41 @@ def test_c1_full_size_window_parity(c1_full):
     print("[c1 full] fast vs fp64: max %.3e mean %.3e" % (e.max(), e.mean()), flush=True)
     assert e.max() <= FAST_TOL
     assert np.all(g64 >= 0) and np.all(g64 <= 1)
+
+
+_IMAGE_IN_CHILD = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from simulated_tem_amd import imaging, synth
+imaging.init(0)
+cfg = synth.scaled("c0", (10, 9, 8), (26.1, 24.7, 22.9))
+cfg.update(sg=[0.01], density=True, g=[(1, -1, 0)], k=(1, 1, 1), n_tomo=1)
+x, box = synth.make_atoms(cfg)
+x, conds = synth.conditions(cfg, x, box)
+im = conds[0][0]
+np.save(sys.argv[2], im.computeD2BI_img(x, 0.005, 9, mode="fast"))
+"""
+
+
+def test_fast_images_identical_across_processes(tmp_path):
+    """A FAST image does not depend on the process that computes it: two
+    fresh processes image the same condition bit for bit alike (the voxel
+    kernel variant and the integrator are fixed choices, the sums have a
+    fixed order)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = []
+    for i in range(2):
+        out = tmp_path / ("img%d.npy" % i)
+        env = dict(os.environ)
+        env.pop("D2BI_FAST_VARIANT", None)
+        env.pop("D2BI_FAST_TUNE", None)
+        print("[process] %d" % i, flush=True)
+        r = subprocess.run([sys.executable, "-c", _IMAGE_IN_CHILD, root, str(out)], env=env,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append(np.load(out))
+    assert outs[0].size > 0 and np.all(np.isfinite(outs[0]))
+    assert np.array_equal(outs[0], outs[1])
