@@ -82,7 +82,7 @@ def batch_bench(args, imaging, synth):
         imaged += p.Mx * p.My * p.Mz * npr
     dev = imaging.DeviceBuffer((imaging.batch_image_size(batch),))
     t0 = time.perf_counter()
-    imaging.computeD2BI_img_batch(batch, mode="fast", out=dev)       # plans, tunes, captures
+    imaging.computeD2BI_img_batch(batch, mode="fast", out=dev)       # plans, captures
     log("[bench] batch first call (capture) %.2f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
     for _ in range(args.batch_steps):

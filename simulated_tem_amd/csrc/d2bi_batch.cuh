@@ -263,7 +263,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         } else {
             CHK(w.f32_field.ensure(sizeof(float4) * max_vox));
         }
-        // FAST kernel variants: tune unseen shapes eagerly (outside the graph)
+        // FAST kernel variants (fixed, forced, or with D2BI_FAST_TUNE=1 tuned
+        // eagerly for unseen shapes, outside the graph)
         for (int i = 0; i < nRec && mode == D2BI_MODE_FAST; i++) {
             int v = known_variant(&C[i].plan, C[i].dens);
             if (v < 0) {
@@ -274,7 +275,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             }
             C[i].variant = v;
         }
-        // the eager tuning may have grown workspace buffers: capture after it
+        // eager tuning may have grown workspace buffers: capture after it
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int rc = D2BI_OK;

@@ -234,7 +234,7 @@ int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t 
     Binned b;
     CHK(run_binning(d_x, nAtoms, BP, &b, s));
     // FAST without count and r2min runs the imaging kernel itself (the
-    // variant D2BI_FAST_VARIANT names, or the tuned one)
+    // variant imaging uses)
     const bool diag = mode == D2BI_MODE_FP64 || count || r2min;
     CHK(run_voxel(rec, &plan, tab, ax, mode, diag, nullptr, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -262,7 +262,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         return ms;
     });
     if (variant < 0) return variant;
-    HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches counted too
+    HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches (D2BI_FAST_TUNE=1) counted too
     if (T) CHK(T->record(2));
     CHK(run_variant(variant));
     HIPCHK(hipGetLastError());
