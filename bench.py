@@ -70,15 +70,26 @@ def batch_bench(args, imaging, synth):
     (value_image_to_host, PCIe included).  Plain FP64 = eager per-condition
     calls on a subset."""
     import numpy as np
-    cfg = dict(synth.CONFIGS[args.batch])
     log("[bench] batch workload", args.batch)
-    x, box = synth.make_atoms(cfg)
-    x, conds = synth.conditions(cfg, x, box)
-    at = imaging.Atoms(x)
-    batch = [(im, at, pa, npr) for (im, pa, npr) in conds]
+    if args.batch == "c4":
+        # the uneven batch: 64 seeded configurations, each under its own
+        # g-vectors, voxel step and precession points
+        data = synth.c4_conditions(synth.c4_configs())
+        flat = []
+        for x, conds in data:
+            at = imaging.Atoms(x)
+            flat.extend((im, at, pa, npr, len(x)) for (im, pa, npr) in conds)
+    else:
+        cfg = dict(synth.CONFIGS[args.batch])
+        x, box = synth.make_atoms(cfg)
+        x, conds = synth.conditions(cfg, x, box)
+        at = imaging.Atoms(x)
+        flat = [(im, at, pa, npr, len(x)) for (im, pa, npr) in conds]
+    conds = [(im, pa, npr) for (im, at, pa, npr, n) in flat]
+    batch = [(im, at, pa, npr) for (im, at, pa, npr, n) in flat]
     imaged = 0
-    for im, pa, npr in conds:
-        p = im.plan(len(x), pa, npr)
+    for im, at, pa, npr, n in flat:
+        p = im.plan(n, pa, npr)
         imaged += p.Mx * p.My * p.Mz * npr
     dev = imaging.DeviceBuffer((imaging.batch_image_size(batch),))
     t0 = time.perf_counter()
@@ -100,8 +111,8 @@ def batch_bench(args, imaging, synth):
     errs, imaged64 = [], 0
     t64 = 0.0
     for i in idx:
-        im, pa, npr = conds[i]
-        p = im.plan(len(x), pa, npr)
+        im, at, pa, npr, n = flat[i]
+        p = im.plan(n, pa, npr)
         t0 = time.perf_counter()
         g64 = im.computeD2BI_img(at, pa, npr, mode="fp64")
         t64 += time.perf_counter() - t0
@@ -110,7 +121,7 @@ def batch_bench(args, imaging, synth):
     fp64_value = imaged64 / t64
     log("[bench] batch fp64 (plain, %d conditions): %.3e imaged voxels/s, max err %.3e"
         % (len(idx), fp64_value, max(errs)))
-    return {"workload": args.batch, "conditions": len(conds), "n_prec": conds[0][2],
+    return {"workload": args.batch, "conditions": len(conds), "n_prec": max(c[2] for c in conds),
             "imaged_voxels_per_batch": imaged, "value": value, "unit": "imaged voxels/s",
             "s_per_batch": s_per_batch, "steps": args.batch_steps,
             "value_image_to_host": imaged / s_host,
@@ -145,7 +156,7 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=384,
                     help="side of the column window the CPU baseline images (~10 s on 16 threads for c1)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch", default="c3", help="batched workload ('' to skip)")
+    ap.add_argument("--batch", default="c3", help="batched workload: c3 (tilt series), c4 (uneven), '' to skip")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-fp64", type=int, default=3, help="conditions timed in plain FP64")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
