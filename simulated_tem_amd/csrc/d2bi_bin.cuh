@@ -401,10 +401,9 @@ struct Binned {
     int max_per_cell = 0;
 };
 
-// Pass 4 (sort within cells, then gather by slot).
-static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s)
+// Pass 4 (sort within cells, then gather by slot), on workspace w.
+static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s, Workspace& w)
 {
-    Workspace& w = g.ws;
     const int nb = (ncell + 255) / 256;         // blk_max holds >= nb ints (sized with cell_start)
     hipLaunchKernelGGL(k_bin_sort, dim3(nb), dim3(256), 0, s, ncell, w.cell_start.as<int>(), w.cell_ids.as<int>(),
                        w.blk_max.as<int>());
@@ -482,7 +481,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
                            w.cell_ids.as<int>());
         HIPCHK(hipGetLastError());
     }
-    launch_bin_finish(ncell, nr, P, w.counters.as<int>(), s);
+    launch_bin_finish(ncell, nr, P, w.counters.as<int>(), s, w);
     HIPCHK(hipGetLastError());
     out->nr = nr;
     out->ncell = ncell;

@@ -338,10 +338,11 @@ static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, d
 
 // accumulate: add each column's sum over precession points to d_img;
 // otherwise write sum / nPrec (div = 0) or sum / div.
+// The FP64 field is read from workspace w (the library's by default).
 static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double* d_img, hipStream_t s,
-                                 int accumulate = 0, double div = 0.0)
+                                 int accumulate = 0, double div = 0.0, Workspace* wp = nullptr)
 {
-    Workspace& w = g.ws;
+    Workspace& w = wp ? *wp : g.ws;
     const int64_t plane = (int64_t)p->Mx * p->My;
     IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa, accumulate,
                 div == 0.0 ? (double)p->nPrec : div};
