@@ -177,7 +177,9 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
     key_append(key, &nb, sizeof nb);
     for (int i = 0; i < nRec; i++) {
         key_append(key, &recs[i], sizeof(d2bi_record));
-        key_append(key, &atoms[i]->x, sizeof(double*));
+        // the graph holds each set's point counts: key on the upload, not on
+        // addresses that a later upload may reuse
+        key_append(key, &atoms[i]->serial, sizeof(uint64_t));
         key_append(key, &atoms[i]->n, sizeof(int64_t));
         key_append(key, &precAngle[i], sizeof(double));
         key_append(key, &nPrec[i], sizeof(int32_t));

@@ -146,6 +146,9 @@ static int pinned(size_t bytes, void** out)
 struct d2bi_atoms_s {
     double* x = nullptr;        // 3*n doubles, AoS
     int64_t n = 0;
+    // unique per upload: a freed set's device address (and this struct's)
+    // can come back for the next upload, with other positions
+    uint64_t serial = 0;
 };
 
 namespace d2bi {

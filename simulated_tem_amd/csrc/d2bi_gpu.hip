@@ -89,8 +89,10 @@ int d2bi_gpu_atoms_upload(const double* x, int64_t ld, int64_t nAtoms, d2bi_atom
 {
     CHK(need_init());
     if (!out || nAtoms < 0 || (nAtoms > 0 && (!x || ld < 3))) return seterr(D2BI_ERR_ARG, "bad atom array");
+    static uint64_t serial = 0;
     d2bi_atoms_s* a = new d2bi_atoms_s;
     a->n = nAtoms;
+    a->serial = ++serial;
     size_t bytes = sizeof(double) * 3 * std::max<int64_t>(nAtoms, 1);
     if (hipMalloc(&a->x, bytes) != hipSuccess) {
         delete a;

@@ -336,6 +336,28 @@ def test_batch_graph_replays_equal_eager(mode):
     assert rc != 0                                   # host memory is refused
 
 
+@pytest.mark.parametrize("mode", ["fast", "fp64"])
+def test_batch_after_reupload_same_count(mode):
+    """The next frame of a series: one atom set freed and another of the same
+    count uploaded (which may get the freed set's addresses back).  The batch
+    must not replay the graph of the first set, whose point counts it holds:
+    the new frame's images equal its eager images."""
+    x, im = _cond("c0", cells=(10, 10, 10))
+    at = imaging.Atoms(x)
+    st1 = imaging.D2biStats()
+    first = imaging.computeD2BI_img_batch([(im, at, 0.005, 3)], mode=mode, stats=st1)[0]
+    assert np.array_equal(first, im.computeD2BI_img(at, 0.005, 3, mode=mode))
+    del at
+    c = x.mean(axis=0)
+    x2 = c + 0.97 * (x - c)                          # same count, fewer points in the cell box
+    at2 = imaging.Atoms(x2)
+    st2 = imaging.D2biStats()
+    second = imaging.computeD2BI_img_batch([(im, at2, 0.005, 3)], mode=mode, stats=st2)[0]
+    assert st2.n_points != st1.n_points
+    assert np.array_equal(second, im.computeD2BI_img(at2, 0.005, 3, mode=mode))
+    assert not np.array_equal(second, first)
+
+
 def test_uneven_batch_c4_style():
     """A small uneven batch like c4: mixed M (3, 4, 6), zone axes 001 and
     01-1, g in {110, 1-10, 200, 211}, 1-8 precession points, density on and
