@@ -143,7 +143,8 @@ int d2bi_gpu_image_dev_out(const d2bi_record* rec, d2bi_atoms atoms, double prec
         (void)hipGetLastError();                 // a failed query must not linger as the last error
         return seterr(D2BI_ERR_ARG, "d_img is not device memory");
     }
-    if (attr.type != hipMemoryTypeDevice) return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    if (attr.type != hipMemoryTypeDevice || attr.device != g.device)
+        return seterr(D2BI_ERR_ARG, "d_img is not memory of device %d", g.device);
     if (stats) std::memset(stats, 0, sizeof *stats);
     d2bi_plan plan;
     CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
@@ -211,7 +212,8 @@ int d2bi_gpu_image_batch_dev_out(const d2bi_record* recs, const d2bi_atoms* atom
         (void)hipGetLastError();                 // a failed query must not linger as the last error
         return seterr(D2BI_ERR_ARG, "d_img is not device memory");
     }
-    if (attr.type != hipMemoryTypeDevice) return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    if (attr.type != hipMemoryTypeDevice || attr.device != g.device)
+        return seterr(D2BI_ERR_ARG, "d_img is not memory of device %d", g.device);
     return run_batch(recs, atoms, nRec, precAngle, nPrec, mode, d_img, true, stats);
 }
 
@@ -338,10 +340,12 @@ int d2bi_gpu_lc_query(const double a[9], int32_t Nx, int32_t Ny, int32_t Nz, con
             return rc_;
         }
         hipError_t e = hipMemcpy(dq.p, q, sizeof(double) * 3 * nQ, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-        hipLaunchKernelGGL(k_lc_query, dim3((unsigned)((nQ + 255) / 256)), dim3(256), 0, s, Q, nQ, dq.as<double>(),
-                           w.cell_start.as<int>(), w.slot_id.as<int>(), w.slot_frac.as<double4>(), dn.as<int>(),
-                           did.as<int>(), dr2.as<double>());
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_lc_query, dim3((unsigned)((nQ + 255) / 256)), dim3(256), 0, s, Q, nQ,
+                               dq.as<double>(), w.cell_start.as<int>(), w.slot_id.as<int>(),
+                               w.slot_frac.as<double4>(), dn.as<int>(), did.as<int>(), dr2.as<double>());
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e == hipSuccess) e = hipMemcpy(n, dn.p, sizeof(int) * nQ, hipMemcpyDeviceToHost);
         if (e == hipSuccess && maxNeigh > 0) e = hipMemcpy(id, did.p, sizeof(int) * nQ * maxNeigh, hipMemcpyDeviceToHost);
