@@ -273,8 +273,31 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             max_cell = std::max<int64_t>(max_cell, C[i].ncell + 1);
             max_vox = std::max(max_vox, C[i].nvox);
         }
-        const int nbr = std::min(batch_branches(), nRec);
-        // every branch gets a workspace sized for the largest condition
+        int nbr = std::min(batch_branches(), nRec);
+        // every branch gets a workspace sized for the largest condition; the
+        // branches beyond the first run only where their workspaces fit in
+        // free device memory (with a margin), so that a batch one workspace
+        // can hold does not fail for want of four
+        if (nbr > 1) {
+            const double per = (double)sizeof(double) * 17 * max_nr + 8.0 * max_cell +
+                               (mode == D2BI_MODE_FP64 ? 24.0 : 16.0) * max_vox;
+            const double need = 1.125 * per + (1 << 20);           // Buf headroom, small buffers
+            size_t freeb = 0, totalb = 0;
+            const char* e = std::getenv("D2BI_BATCH_MEM");         // free bytes to assume (tests)
+            if (e && *e) freeb = (size_t)std::strtoull(e, nullptr, 10);
+            else if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+                (void)hipGetLastError();
+                freeb = 0;
+            }
+            int fit = 1;
+            double room = 0.85 * (double)freeb;
+            for (int k = 1; k < nbr; k++) {
+                room += (double)B.ws[k - 1].bytes();               // reused, not allocated again
+                if (room < need * k) break;
+                fit = k + 1;
+            }
+            nbr = fit;
+        }
         for (int k = 0; k < nbr; k++) {
             Workspace& w = B.branch_ws(k);
             CHK(w.pt_cell.ensure(sizeof(int) * max_nr));

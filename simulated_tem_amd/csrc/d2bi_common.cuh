@@ -88,14 +88,25 @@ struct Workspace {
     // pinned host staging
     void* pinned = nullptr;
     size_t pinned_cap = 0;
-    void release_all()
+    template <class F> void each(F f)
     {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_p32, &pt_sn32,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
                       &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_k};
-        for (Buf* b : all) b->release();
+        for (Buf* b : all) f(*b);
+    }
+    // device bytes held
+    size_t bytes()
+    {
+        size_t n = 0;
+        each([&](Buf& b) { n += b.cap; });
+        return n;
+    }
+    void release_all()
+    {
+        each([](Buf& b) { b.release(); });
         if (pinned) (void)hipHostFree(pinned);
         pinned = nullptr;
         pinned_cap = 0;

@@ -358,6 +358,33 @@ def test_batch_after_reupload_same_count(mode):
     assert not np.array_equal(second, first)
 
 
+def test_batch_branches_fit_free_memory(monkeypatch):
+    """Each concurrent branch of a batch holds a workspace sized for the
+    largest condition; extra branches run only where free device memory holds
+    them.  With no free memory assumed (D2BI_BATCH_MEM=0) and no workspaces
+    left from earlier calls, the batch runs on one branch instead of failing,
+    and its images equal the four-branch images bit for bit."""
+    x, im1 = _cond("c0", cells=(10, 10, 10))
+    _, im2 = _cond("c0", cells=(10, 10, 10), sg=0.015, density=True)
+    x3, im3 = _cond("c1", cells=(12, 9, 7))
+
+    def run():
+        at, at3 = imaging.Atoms(x), imaging.Atoms(x3)
+        conds = [(im1, at, 0.005, 3), (im2, at, 0.005, 1), (im3, at3, 0.004, 6), (im1, at3, 0.0, 1)]
+        return imaging.computeD2BI_img_batch(conds, mode="fast")
+
+    ref = run()
+    imaging.finalize()                               # drop every workspace
+    monkeypatch.setenv("D2BI_BATCH_MEM", "0")
+    try:
+        for rep in range(2):
+            for a, b in zip(run(), ref):
+                assert np.array_equal(a, b), rep
+    finally:
+        monkeypatch.delenv("D2BI_BATCH_MEM")
+        imaging.finalize()
+
+
 def test_uneven_batch_c4_style():
     """A small uneven batch like c4: mixed M (3, 4, 6), zone axes 001 and
     01-1, g in {110, 1-10, 200, 211}, 1-8 precession points, density on and
