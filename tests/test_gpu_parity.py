@@ -358,6 +358,28 @@ def test_batch_after_reupload_same_count(mode):
     assert not np.array_equal(second, first)
 
 
+@pytest.mark.parametrize("mode", ["fast", "fp64"])
+def test_batch_empty_and_sparse_atom_sets(mode):
+    """A batch whose atom sets include an empty one, one atom far outside the
+    supercell (no point in any link cell) and a single atom: every image
+    equals its eager image, and the first two image to zero."""
+    x, im = _cond("c0", cells=(6, 6, 6))
+    empty = imaging.Atoms(np.zeros((0, 3)))
+    far = imaging.Atoms(np.array([[1.0e6, 1.0e6, 1.0e6]]))
+    one = imaging.Atoms(np.array([[9.0, 9.5, 10.0]]))
+    full = imaging.Atoms(x)
+    conds = [(im, empty, 0.005, 3), (im, far, 0.0, 1), (im, one, 0.0, 1), (im, full, 0.005, 2),
+             (im, empty, 0.0, 9)]
+    eager = [c[0].computeD2BI_img(c[1], c[2], c[3], mode=mode) for c in conds]
+    for rep in range(2):
+        out = imaging.computeD2BI_img_batch(conds, mode=mode)
+        for a, b in zip(out, eager):
+            assert np.array_equal(a, b), rep
+    for i in (0, 1, 4):
+        assert np.all(out[i] == 0.0)
+    assert np.any(out[2] != 0.0) and np.any(out[3] != 0.0)
+
+
 def test_batch_branches_fit_free_memory(monkeypatch):
     """Each concurrent branch of a batch holds a workspace sized for the
     largest condition; extra branches run only where free device memory holds
