@@ -146,6 +146,23 @@ def calibrate_copy(mib, reps=3):
                       "GB_per_s": 8 * n / dt / 1e9}), flush=True)
 
 
+def blur_only(n, reps=5, t=3.16):
+    """Blur a seeded n x n image reps times (d2bi_gpu_blur: host image in,
+    host image out) for a kernel trace of k_blur_x / k_blur_y; t = 3.16 px is
+    c1's blur (2.5 A over 0.79 A pixels)."""
+    import numpy as np
+    from simulated_tem_amd import imaging
+    imaging.init(0)
+    f = np.random.default_rng(5).random((n, n))
+    imaging.gaussianBlurImage(f, t)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        imaging.gaussianBlurImage(f, t)
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"blur_pixels": n * n, "t": t, "reps": reps, "ms_per_call_host_io": 1e3 * dt,
+                      "kernel_bytes_per_pass": 16 * n * n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +176,8 @@ def main():
     ap.add_argument("--batch", default="c3", help="batched workload: c3 (tilt series), c4 (uneven), '' to skip")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-fp64", type=int, default=3, help="conditions timed in plain FP64")
+    ap.add_argument("--blur", type=int, default=0, metavar="N",
+                    help="only blur a seeded N x N image a few times (for a kernel trace of the blur)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -170,6 +189,9 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.blur:
+        blur_only(args.blur)
         return
 
     import numpy as np

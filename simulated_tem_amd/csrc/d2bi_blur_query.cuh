@@ -18,12 +18,16 @@ static constexpr int BLUR_TILE = 256;
 static constexpr int BLUR_MAXK = 256;
 
 // x pass of gaussianBlurImage (d2bi.f90:1577-1599): row segments staged in
-// LDS with their halo; the kernel weights come from the host (exp in FP64).
+// LDS with their halo, and the kernel weights (exp in FP64 on the host) in
+// LDS.  Every lane runs the taps t = -nk..nk, skipping those outside the row,
+// so each output adds its terms in the reference's order (jj ascending).  The
+// edge weight sums W_x, W_y depend only on the position and come from the host.
 __global__ __launch_bounds__(BLUR_TILE) void k_blur_x(BlurParams B, const double* __restrict__ fin,
                                                       const double* __restrict__ kern,
-                                                      double* __restrict__ tmp, double* __restrict__ wsx)
+                                                      double* __restrict__ tmp)
 {
     __shared__ double row[BLUR_TILE + 2 * BLUR_MAXK];
+    __shared__ double ks[BLUR_MAXK + 1];
     const int iy = blockIdx.y;
     const int x0 = blockIdx.x * BLUR_TILE;
     const int lo = x0 - B.nk, n = BLUR_TILE + 2 * B.nk;
@@ -31,36 +35,60 @@ __global__ __launch_bounds__(BLUR_TILE) void k_blur_x(BlurParams B, const double
         const int gx = lo + t;
         row[t] = (gx >= 0 && gx < B.nx) ? fin[(int64_t)gx + (int64_t)B.nx * iy] : 0.0;
     }
+    for (int t = threadIdx.x; t <= B.nk; t += BLUR_TILE) ks[t] = kern[t];
     __syncthreads();
     const int ix = x0 + threadIdx.x;
     if (ix >= B.nx) return;
-    const int j0 = max(0, ix - B.nk), j1 = min(B.nx - 1, ix + B.nk);
-    double wf = 0.0, ws = 0.0;
-    for (int jj = j0; jj <= j1; jj++) {
-        const double ww = kern[abs(jj - ix)];
-        wf = wf + ww * row[jj - lo];
-        ws = ws + ww;
+    const double* r = row + (ix - lo);                  // r[t] = f(ix + t)
+    double wf = 0.0;
+    if (lo >= 0 && x0 + BLUR_TILE - 1 + B.nk < B.nx) {  // every tap inside the row (uniform)
+        for (int t = -B.nk; t < 0; t++) wf = wf + ks[-t] * r[t];
+        for (int t = 0; t <= B.nk; t++) wf = wf + ks[t] * r[t];
+    } else {
+        for (int t = -B.nk; t <= B.nk; t++) {
+            const int jj = ix + t;
+            if (jj >= 0 && jj < B.nx) wf = wf + ks[t < 0 ? -t : t] * r[t];
+        }
     }
     tmp[(int64_t)ix + (int64_t)B.nx * iy] = wf;
-    if (iy == 0) wsx[ix] = 1 / ws;
 }
 
-// y pass (d2bi.f90:1605-1619): out = sum_y k * tmp * (1/W_x) / W_y.
+// y pass (d2bi.f90:1605-1619): out = sum_y k * tmp * (1/W_x) / W_y.  A lane
+// finishes BLUR_RY consecutive rows of its column and reads each input row of
+// their window once (the rows of a wave are the same, so the tap tests are
+// uniform); each output still adds its terms with jj ascending, in the
+// reference's order.
+static constexpr int BLUR_RY = 8;
+
 __global__ __launch_bounds__(256) void k_blur_y(BlurParams B, const double* __restrict__ tmp,
                                                 const double* __restrict__ kern,
-                                                const double* __restrict__ wsx, double* __restrict__ out)
+                                                const double* __restrict__ wx_inv,
+                                                const double* __restrict__ wy, double* __restrict__ out)
 {
+    __shared__ double ks[BLUR_MAXK + 1];
+    for (int t = threadIdx.x; t <= B.nk; t += 256) ks[t] = kern[t];
+    __syncthreads();
     const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (ix >= B.nx || iy >= B.ny) return;
-    const int j0 = max(0, iy - B.nk), j1 = min(B.ny - 1, iy + B.nk);
-    double wf = 0.0, ws = 0.0;
-    for (int jj = j0; jj <= j1; jj++) {
-        const double ww = kern[abs(jj - iy)];
-        wf = wf + ww * tmp[(int64_t)ix + (int64_t)B.nx * jj];
-        ws = ws + ww;
+    // the wave's first row, held in a scalar register
+    const int y0 = __builtin_amdgcn_readfirstlane((blockIdx.y * 4 + (threadIdx.x >> 6)) * BLUR_RY);
+    if (ix >= B.nx || y0 >= B.ny) return;
+    double wf[BLUR_RY];
+#pragma unroll
+    for (int k = 0; k < BLUR_RY; k++) wf[k] = 0.0;
+    const int r0 = max(0, y0 - B.nk), r1 = min(B.ny - 1, y0 + BLUR_RY - 1 + B.nk);
+#pragma unroll 4
+    for (int r = r0; r <= r1; r++) {
+        const double v = tmp[(int64_t)ix + (int64_t)B.nx * r];
+#pragma unroll
+        for (int k = 0; k < BLUR_RY; k++) {
+            const int t = r - (y0 + k);
+            if (t >= -B.nk && t <= B.nk) wf[k] = wf[k] + ks[t < 0 ? -t : t] * v;
+        }
     }
-    out[(int64_t)ix + (int64_t)B.nx * iy] = (wf * wsx[ix]) / ws;
+    const double wxi = wx_inv[ix];
+#pragma unroll
+    for (int k = 0; k < BLUR_RY; k++)
+        if (y0 + k < B.ny) out[(int64_t)ix + (int64_t)B.nx * (y0 + k)] = (wf[k] * wxi) / wy[y0 + k];
 }
 
 // ------------------------------------------------------------------ generic link-cell query

@@ -84,7 +84,7 @@ struct Workspace {
     Buf f64_re, f64_im, f64_rho, f32_field;
     Buf cnt, r2m;
     Buf img;
-    Buf blur_in, blur_tmp, blur_out, blur_w, blur_k;
+    Buf blur_in, blur_tmp, blur_out, blur_w, blur_wy, blur_k;
     // pinned host staging
     void* pinned = nullptr;
     size_t pinned_cap = 0;
@@ -94,7 +94,7 @@ struct Workspace {
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
                       &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
-                      &blur_out, &blur_w, &blur_k};
+                      &blur_out, &blur_w, &blur_wy, &blur_k};
         for (Buf* b : all) f(*b);
     }
     // device bytes held

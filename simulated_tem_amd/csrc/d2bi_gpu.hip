@@ -367,21 +367,36 @@ int d2bi_gpu_blur(const double* in, int32_t nx, int32_t ny, double t, double* ou
     const double i2s2 = 1 / (2 * t * t);
     kern[0] = 1.0;
     for (int i = 1; i <= nk; i++) kern[i] = std::exp((double)(-i * i) * i2s2);
+    // edge weight sums W(i) = sum of kern[|j - i|] over the window inside the
+    // image, j ascending (d2bi.f90:1585-1590, 1609-1613): 1 / W_x per column,
+    // W_y per row
+    auto wsum = [&](int i, int n) {
+        double ws = 0.0;
+        for (int j = std::max(0, i - nk); j <= std::min(n - 1, i + nk); j++) ws = ws + kern[std::abs(j - i)];
+        return ws;
+    };
+    std::vector<double> wx_inv(nx), wy(ny);
+    for (int i = 0; i < nx; i++) wx_inv[i] = 1 / wsum(i, nx);
+    for (int i = 0; i < ny; i++) wy[i] = wsum(i, ny);
     Workspace& w = g.ws;
     const int64_t np = (int64_t)nx * ny;
     CHK(w.blur_in.ensure(sizeof(double) * np));
     CHK(w.blur_tmp.ensure(sizeof(double) * np));
     CHK(w.blur_out.ensure(sizeof(double) * np));
     CHK(w.blur_w.ensure(sizeof(double) * nx));
+    CHK(w.blur_wy.ensure(sizeof(double) * ny));
     CHK(w.blur_k.ensure(sizeof(double) * (nk + 1)));
     hipStream_t s = g.stream;
     HIPCHK(hipMemcpyAsync(w.blur_in.p, in, sizeof(double) * np, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(w.blur_k.p, kern.data(), sizeof(double) * (nk + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.blur_w.p, wx_inv.data(), sizeof(double) * nx, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.blur_wy.p, wy.data(), sizeof(double) * ny, hipMemcpyHostToDevice, s));
     BlurParams B{nx, ny, nk};
     hipLaunchKernelGGL(k_blur_x, dim3((nx + BLUR_TILE - 1) / BLUR_TILE, ny), dim3(BLUR_TILE), 0, s, B,
-                       w.blur_in.as<double>(), w.blur_k.as<double>(), w.blur_tmp.as<double>(), w.blur_w.as<double>());
-    hipLaunchKernelGGL(k_blur_y, dim3((nx + 63) / 64, (ny + 3) / 4), dim3(256), 0, s, B, w.blur_tmp.as<double>(),
-                       w.blur_k.as<double>(), w.blur_w.as<double>(), w.blur_out.as<double>());
+                       w.blur_in.as<double>(), w.blur_k.as<double>(), w.blur_tmp.as<double>());
+    hipLaunchKernelGGL(k_blur_y, dim3((nx + 63) / 64, (ny + 4 * BLUR_RY - 1) / (4 * BLUR_RY)), dim3(256), 0, s, B,
+                       w.blur_tmp.as<double>(), w.blur_k.as<double>(), w.blur_w.as<double>(), w.blur_wy.as<double>(),
+                       w.blur_out.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, w.blur_out.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

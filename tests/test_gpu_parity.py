@@ -192,7 +192,10 @@ def test_image_aperture(n_app, ap_angle, density, fast_variant):
 
 def test_blur_matches_oracle():
     rng = np.random.default_rng(7)
-    for (ny, nx, t) in [(94, 94, 3.1446), (800, 800, 3.16), (33, 517, 0.4), (300, 7, 12.3)]:
+    # windows cut by both edges, one row or column, a height that is no
+    # multiple of the y pass's rows per lane, and the widest kernel (nk = 256)
+    for (ny, nx, t) in [(94, 94, 3.1446), (800, 800, 3.16), (33, 517, 0.4), (300, 7, 12.3),
+                        (13, 90, 5.0), (1, 40, 2.0), (40, 1, 2.0), (600, 20, 51.0)]:
         img = rng.uniform(size=(ny, nx))
         got = imaging.gaussianBlurImage(img, t)
         want = orc.blur(img, t)
