@@ -105,6 +105,9 @@ typedef struct d2bi_stats {
     double  ms_voxel;           /* device time of voxelisation (ms)                    */
     double  ms_integrate;       /* device time of column integration (ms)              */
     double  ms_total;           /* device time of the whole call (ms)                  */
+    int64_t branch_bytes;       /* batches: device bytes of one branch's workspace     */
+    int32_t n_branches;         /* batches: concurrent graph branches used             */
+    int32_t pad0;
 } d2bi_stats;
 
 /* ---------------------------------------------------------------- runtime */

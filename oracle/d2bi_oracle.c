@@ -13,7 +13,8 @@
  * contract to FMA; the oracle fixes the un-contracted order instead, and the
  * GPU FP64 mode follows the oracle.
  *
- * Pinning (what checks each function; the tests cite the same list):
+ * Pinning (what checks each function; tests/test_oracle_kat.py and
+ * tests/test_oracle_pins.py):
  *   orc_wavelength/velocity   KAT testLib_RelativisticElectrons.f90:27-29
  *   orc_det3/orc_inverse3     closed form on diagonal / random matrices (M*N = I)
  *   orc_lc_* (add, neighbours) KAT testLib_LinkCell3D.f90:271-290 (ids, order,
@@ -22,11 +23,17 @@
  *   orc_integrate_column      closed form intensityAtDepthZ
  *                              (Lib_DiffractionConditions.f90:1209-1258),
  *                              norm conservation, empty column -> 0
- *   orc_voxel (S, rho)        single atom -> e^{i g.r}, empty -> 0, perfect
- *                              crystal -> constant phase; smoothstep values
- *   orc_find_gdotr            brute force replica/box enumeration
- *   orc_compute_img           golden image Data/test/...ng1.00.dat (approx),
- *                              perfect crystal vs intensityAtDepthZ
+ *   voxel_sum (x, rho)        perfect bcc crystal with h+k+l even -> one
+ *                              constant unit phase factor on every voxel;
+ *                              smoothstep values of rho
+ *   orc_find_gdotr            brute force: every image of every atom over
+ *                              5x5x5 supercell replicas, rotated, offset and
+ *                              kept inside the link-cell box, is the oracle's
+ *                              point set (cell coordinates and g.r)
+ *   orc_compute_img           golden image Data/test/...ng1.00.dat (approx);
+ *                              perfect crystal -> every column equals the
+ *                              x = 1 column, within RK4 error (2e-6) of
+ *                              intensityAtDepthZ
  *   orc_blur                  constant image invariance, delta response
  *                              = normalised separable Gaussian, symmetry
  *   orc_plan                  golden .dat header (|g|, scale, image space)

@@ -70,6 +70,7 @@ class D2biStats(C.Structure):
         ("max_per_cell", C.c_int32), ("n_cells", C.c_int32),
         ("ms_bin", C.c_double), ("ms_voxel", C.c_double),
         ("ms_integrate", C.c_double), ("ms_total", C.c_double),
+        ("branch_bytes", C.c_int64), ("n_branches", C.c_int32), ("pad0", C.c_int32),
     ]
 
 

@@ -47,6 +47,8 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     std::vector<int64_t> nr;
     int64_t total_plane = 0;
+    int nbr = 1;                                // branches of the captured graph
+    int64_t branch_bytes = 0;                   // device bytes of one branch's workspace
     Workspace ws[MAX_BRANCH - 1];               // branches 1..MAX_BRANCH-1
     hipStream_t stream[MAX_BRANCH - 1] = {};
     hipEvent_t fork = nullptr, join[MAX_BRANCH - 1] = {};
@@ -154,6 +156,59 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, Workspace& w
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
     return launch_integrate_fast(p, c.dens, w.f32_field.p, d_img, s);
+}
+
+// Sizes of one branch's workspace: the largest condition of the batch.
+struct BranchSizes {
+    int mode;
+    int64_t max_nr, max_cell, max_vox, max_scan;
+};
+
+// The buffers of a branch workspace that a batch uses, with their sizes.
+template <class F>
+static void branch_buffers(Workspace& w, const BranchSizes& z, F f)
+{
+    f(w.pt_cell, sizeof(int) * z.max_nr);
+    f(w.pt_slot, sizeof(int) * z.max_nr);
+    f(w.pt_frac, sizeof(double) * 3 * z.max_nr);
+    f(w.pt_gdotr, sizeof(double) * z.max_nr);
+    f(w.pt_p32, sizeof(float4) * z.max_nr);
+    f(w.pt_sn32, sizeof(float) * z.max_nr);
+    f(w.cell_ids, sizeof(int) * z.max_nr);
+    f(w.slot_id, sizeof(int) * z.max_nr);
+    f(w.slot_frac, sizeof(double4) * z.max_nr);
+    f(w.slot_cs64, sizeof(double2) * z.max_nr);
+    f(w.slot_p32, sizeof(float4) * z.max_nr);
+    f(w.slot_sn32, sizeof(float) * z.max_nr);
+    f(w.cell_cnt, sizeof(int) * z.max_cell);
+    f(w.cell_start, sizeof(int) * z.max_cell);
+    f(w.blk_max, sizeof(int) * (z.max_cell / 256 + 2));
+    if (z.mode == D2BI_MODE_FP64) {
+        f(w.f64_re, sizeof(double) * z.max_vox);
+        f(w.f64_im, sizeof(double) * z.max_vox);
+        f(w.f64_rho, sizeof(double) * z.max_vox);
+    } else {
+        f(w.f32_field, sizeof(float4) * z.max_vox);
+    }
+    f(w.guard_part, sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE);
+    f(w.scan_tmp, sizeof(int) * z.max_scan);
+}
+
+// Device bytes that sizing w for the batch would add.
+static double branch_growth(Workspace& w, const BranchSizes& z)
+{
+    double d = 0.0;
+    branch_buffers(w, z, [&](Buf& b, size_t bytes) { d += b.growth(bytes); });
+    return d;
+}
+
+// Device bytes of one branch workspace sized for the batch.
+static int64_t branch_bytes(const BranchSizes& z)
+{
+    Workspace tmp;
+    int64_t n = 0;
+    branch_buffers(tmp, z, [&](Buf&, size_t bytes) { n += (int64_t)Buf::alloc_size(bytes); });
+    return n;
 }
 
 // Batched conditions.  The first call with a given set of inputs plans every
@@ -274,14 +329,13 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             max_vox = std::max(max_vox, C[i].nvox);
         }
         int nbr = std::min(batch_branches(), nRec);
+        const BranchSizes bz{mode, max_nr, max_cell, max_vox, max_scan};
         // every branch gets a workspace sized for the largest condition; the
-        // branches beyond the first run only where their workspaces fit in
-        // free device memory (with a margin), so that a batch one workspace
-        // can hold does not fail for want of four
+        // branches beyond the first run only where what their workspaces
+        // would add (branch 0's growth included) fits in 85 % of the free
+        // device memory, so that a batch one workspace can hold does not
+        // fail for want of four
         if (nbr > 1) {
-            const double per = (double)sizeof(double) * 17 * max_nr + 8.0 * max_cell +
-                               (mode == D2BI_MODE_FP64 ? 24.0 : 16.0) * max_vox;
-            const double need = 1.125 * per + (1 << 20);           // Buf headroom, small buffers
             size_t freeb = 0, totalb = 0;
             const char* e = std::getenv("D2BI_BATCH_MEM");         // free bytes to assume (tests)
             if (e && *e) freeb = (size_t)std::strtoull(e, nullptr, 10);
@@ -289,42 +343,25 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
                 (void)hipGetLastError();
                 freeb = 0;
             }
+            const double room = 0.85 * (double)freeb;
+            double used = branch_growth(B.branch_ws(0), bz);
             int fit = 1;
-            double room = 0.85 * (double)freeb;
             for (int k = 1; k < nbr; k++) {
-                room += (double)B.ws[k - 1].bytes();               // reused, not allocated again
-                if (room < need * k) break;
+                used += branch_growth(B.branch_ws(k), bz);
+                if (used > room) break;
                 fit = k + 1;
             }
             nbr = fit;
         }
         for (int k = 0; k < nbr; k++) {
-            Workspace& w = B.branch_ws(k);
-            CHK(w.pt_cell.ensure(sizeof(int) * max_nr));
-            CHK(w.pt_slot.ensure(sizeof(int) * max_nr));
-            CHK(w.pt_frac.ensure(sizeof(double) * 3 * max_nr));
-            CHK(w.pt_gdotr.ensure(sizeof(double) * max_nr));
-            CHK(w.pt_p32.ensure(sizeof(float4) * max_nr));
-            CHK(w.pt_sn32.ensure(sizeof(float) * max_nr));
-            CHK(w.cell_ids.ensure(sizeof(int) * max_nr));
-            CHK(w.slot_id.ensure(sizeof(int) * max_nr));
-            CHK(w.slot_frac.ensure(sizeof(double4) * max_nr));
-            CHK(w.slot_cs64.ensure(sizeof(double2) * max_nr));
-            CHK(w.slot_p32.ensure(sizeof(float4) * max_nr));
-            CHK(w.slot_sn32.ensure(sizeof(float) * max_nr));
-            CHK(w.cell_cnt.ensure(sizeof(int) * max_cell));
-            CHK(w.cell_start.ensure(sizeof(int) * max_cell));
-            CHK(w.blk_max.ensure(sizeof(int) * (max_cell / 256 + 2)));
-            if (mode == D2BI_MODE_FP64) {
-                CHK(w.f64_re.ensure(sizeof(double) * max_vox));
-                CHK(w.f64_im.ensure(sizeof(double) * max_vox));
-                CHK(w.f64_rho.ensure(sizeof(double) * max_vox));
-            } else {
-                CHK(w.f32_field.ensure(sizeof(float4) * max_vox));
-            }
-            CHK(w.guard_part.ensure(sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE));
-            CHK(w.scan_tmp.ensure(sizeof(int) * max_scan));
+            int rc_ = D2BI_OK;
+            branch_buffers(B.branch_ws(k), bz, [&](Buf& b, size_t bytes) {
+                if (rc_ == D2BI_OK) rc_ = b.ensure(bytes);
+            });
+            CHK(rc_);
         }
+        B.nbr = nbr;
+        B.branch_bytes = branch_bytes(bz);
         // branch streams and the events that fork and join them in the graph
         // (default priority)
         if (!B.fork) HIPCHK(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
@@ -416,6 +453,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[1]);
         st->ms_total += ms;
+        st->n_branches = B.nbr;
+        st->branch_bytes = B.branch_bytes;
     }
     return D2BI_OK;
 }

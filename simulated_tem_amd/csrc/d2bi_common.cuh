@@ -47,8 +47,7 @@ struct Buf {
         if (bytes <= cap && p) return D2BI_OK;
         g_alloc_gen++;
         if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
-        size_t want = std::max<size_t>(bytes, 256);
-        want = want + want / 8;                      // headroom for slightly larger conditions
+        const size_t want = alloc_size(bytes);       // headroom for slightly larger conditions
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) {
             p = nullptr;
@@ -62,6 +61,18 @@ struct Buf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+    }
+    // bytes ensure(bytes) allocates: the request with its headroom
+    static size_t alloc_size(size_t bytes)
+    {
+        const size_t want = std::max<size_t>(bytes, 256);
+        return want + want / 8;
+    }
+    // net device bytes ensure(bytes) would add (it frees the old block first)
+    double growth(size_t bytes) const
+    {
+        if (bytes <= cap && p) return 0.0;
+        return (double)alloc_size(bytes) - (double)cap;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };

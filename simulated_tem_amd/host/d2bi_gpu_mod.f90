@@ -44,6 +44,8 @@ module d2bi_gpu_mod
         integer(c_int64_t) :: n_points, n_guard
         integer(c_int32_t) :: max_per_cell, n_cells
         real(c_double)     :: ms_bin, ms_voxel, ms_integrate, ms_total
+        integer(c_int64_t) :: branch_bytes
+        integer(c_int32_t) :: n_branches, pad0
     end type d2bi_stats
 
     public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_plan_image

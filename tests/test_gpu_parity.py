@@ -19,6 +19,7 @@ import pytest
 from oracle import oracle as orc
 from simulated_tem_amd import abi, imaging, synth
 from simulated_tem_amd.imaging import DynamicalTwoBeamImaging, input_supercell, lattice_a0
+from tests.brute import min_image_field
 
 pytestmark = pytest.mark.gpu
 
@@ -258,15 +259,21 @@ def test_empty_and_single_atom():
     assert np.all(img == 0.0)
     img = im.computeD2BI_img(np.zeros((0, 3)), mode="fast")
     assert np.all(img == 0.0)
-    # one atom: phase factor e^{i g.r} on every voxel within 3 sigma
+    # one atom: its periodic images in the link-cell box are points with
+    # their own g.r; every voxel within 3 sigma of one image has the phase
+    # factor e^{i g.r} of that image (a minimum-image brute force over the
+    # link-cell box, not the link cells), every other voxel x = 0
     one = np.array([[9.0, 9.5, 10.0]])
     ph, rho, cnt, r2m = im.field(one, "fp64")
     oc = orc.Condition(im.rec, one)
-    gd = oc.gdotr()
     assert oc.n_points >= 1
+    bx, bcnt, gap = min_image_field(oc, im.rec)
+    assert gap > 1e-9
     hit = cnt > 0
     assert np.any(hit)
-    assert np.allclose(ph[hit], np.exp(1j * gd[0]), atol=1e-13) or len(gd) > 1
+    assert np.array_equal(cnt, bcnt)
+    assert np.max(np.abs(ph - bx)) < 1e-13
+    assert np.all(np.abs(np.abs(ph[hit]) - 1.0) < 1e-13)
     assert np.all(ph[~hit] == 0)
     img64 = im.computeD2BI_img(one, mode="fp64")
     ref = oc.image()
@@ -385,28 +392,39 @@ def test_batch_empty_and_sparse_atom_sets(mode):
 
 def test_batch_branches_fit_free_memory(monkeypatch):
     """Each concurrent branch of a batch holds a workspace sized for the
-    largest condition; extra branches run only where free device memory holds
-    them.  With no free memory assumed (D2BI_BATCH_MEM=0) and no workspaces
-    left from earlier calls, the batch runs on one branch instead of failing,
-    and its images equal the four-branch images bit for bit."""
+    largest condition (branch_bytes); extra branches run only where what the
+    workspaces add, branch 0's own included, fits in 85 % of the free device
+    memory.  From an empty library (no workspace kept): no free memory gives
+    one branch, free memory for 1.5 workspaces one branch, for 2.5 two, and
+    plenty four; the images are the same bit for bit on any count."""
     x, im1 = _cond("c0", cells=(10, 10, 10))
     _, im2 = _cond("c0", cells=(10, 10, 10), sg=0.015, density=True)
     x3, im3 = _cond("c1", cells=(12, 9, 7))
 
-    def run():
+    def run(mem):
+        imaging.finalize()                           # drop every workspace
+        if mem is None:
+            monkeypatch.delenv("D2BI_BATCH_MEM", raising=False)
+        else:
+            monkeypatch.setenv("D2BI_BATCH_MEM", str(int(mem)))
         at, at3 = imaging.Atoms(x), imaging.Atoms(x3)
         conds = [(im1, at, 0.005, 3), (im2, at, 0.005, 1), (im3, at3, 0.004, 6), (im1, at3, 0.0, 1)]
-        return imaging.computeD2BI_img_batch(conds, mode="fast")
+        st = imaging.D2biStats()
+        out = imaging.computeD2BI_img_batch(conds, mode="fast", stats=st)
+        return out, st.n_branches, st.branch_bytes
 
-    ref = run()
-    imaging.finalize()                               # drop every workspace
-    monkeypatch.setenv("D2BI_BATCH_MEM", "0")
     try:
-        for rep in range(2):
-            for a, b in zip(run(), ref):
-                assert np.array_equal(a, b), rep
+        ref, nb, W = run(1e15)
+        assert nb == 4 and W > 0
+        for mem, want in [(0, 1), (1.5 * W / 0.85, 1), (2.5 * W / 0.85, 2), (1e15, 4)]:
+            out, nb, W2 = run(mem)
+            print("[branches] free %.3e -> %d branches (workspace %d B)" % (mem, nb, W2), flush=True)
+            assert W2 == W
+            assert nb == want, (mem, nb)
+            for a, b in zip(out, ref):
+                assert np.array_equal(a, b), mem
     finally:
-        monkeypatch.delenv("D2BI_BATCH_MEM")
+        monkeypatch.delenv("D2BI_BATCH_MEM", raising=False)
         imaging.finalize()
 
 
