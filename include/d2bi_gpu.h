@@ -31,6 +31,7 @@ extern "C" {
 #define D2BI_MAX_UVW   125      /* Lib_DynamicalTwoBeamImaging.f90:86 uvw(3,125)          */
 #define D2BI_MAX_PREC  256      /* precession points per condition                        */
 #define D2BI_MAX_APERTURE 1024 /* aperture points per condition                         */
+#define D2BI_SCRATCH_BYTES 67108864 /* 64 MiB: device workspace of d2bi_gpu_image_noalloc  */
 
 enum {
     D2BI_OK            =  0,
@@ -226,6 +227,23 @@ int d2bi_gpu_image_dev(const d2bi_record* rec, d2bi_atoms atoms,
  * conditions) and for timing the path without the PCIe copy of the image.
  * The call returns when the image is complete. */
 int d2bi_gpu_image_dev_out(const d2bi_record* rec, d2bi_atoms atoms,
+                           double precAngle, int32_t nPrec, int32_t mode,
+                           double* d_img, d2bi_stats* stats);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc): the same
+ * condition and image as d2bi_gpu_image_dev_out, bit for bit, with every
+ * device buffer the call needs carved from a static block of
+ * D2BI_SCRATCH_BYTES (64 MiB) per device, held from library load to process
+ * end; the call itself calls no hipMalloc, hipFree or pinned-host allocator.
+ * The caller passes the atoms (device-resident) and d_img (device memory,
+ * Mx*My doubles).  A condition whose workspace exceeds the block is refused
+ * with D2BI_ERR_CAPACITY (FAST needs 16 B and FP64 24 B per voxel for the
+ * field, plus link cells and point records: e.g. 80 x 80 x 80 voxels with
+ * 16,000 atoms fit).  Every call on a device shares that device's block and
+ * overwrites it; calls that overlap on one device (from several host
+ * threads) race.  Returns when the image is complete. */
+int d2bi_gpu_image_noalloc(const d2bi_record* rec, d2bi_atoms atoms,
                            double precAngle, int32_t nPrec, int32_t mode,
                            double* d_img, d2bi_stats* stats);
 

@@ -95,7 +95,8 @@ EXPORTED = [
     "d2bi_rotate_to_standard_frame", "d2bi_wavelength", "d2bi_velocity",
     "d2bi_deviation_parameter", "d2bi_aperture_points", "d2bi_aperture_setup",
     "d2bi_gpu_atoms_upload", "d2bi_gpu_atoms_free", "d2bi_gpu_atoms_count",
-    "d2bi_gpu_image", "d2bi_gpu_image_aperture", "d2bi_gpu_image_dev", "d2bi_gpu_image_dev_out", "d2bi_gpu_image_batch", "d2bi_gpu_image_batch_dev_out", "d2bi_gpu_field",
+    "d2bi_gpu_image", "d2bi_gpu_image_aperture", "d2bi_gpu_image_dev", "d2bi_gpu_image_dev_out",
+    "d2bi_gpu_image_noalloc", "d2bi_gpu_image_batch", "d2bi_gpu_image_batch_dev_out", "d2bi_gpu_field",
     "d2bi_gpu_linkcells", "d2bi_gpu_lc_query", "d2bi_gpu_blur", "d2bi_synth_bcc",
 ]
 
@@ -149,6 +150,8 @@ def load():
     L.d2bi_gpu_image_dev.argtypes = [C.POINTER(D2biRecord), C.c_void_p, C.c_double, C.c_int32,
                                      C.c_int32, _dp, C.POINTER(D2biStats)]
     L.d2bi_gpu_image_dev_out.argtypes = [C.POINTER(D2biRecord), C.c_void_p, C.c_double, C.c_int32,
+                                         C.c_int32, C.c_void_p, C.POINTER(D2biStats)]
+    L.d2bi_gpu_image_noalloc.argtypes = [C.POINTER(D2biRecord), C.c_void_p, C.c_double, C.c_int32,
                                          C.c_int32, C.c_void_p, C.POINTER(D2biStats)]
     L.d2bi_gpu_image_batch.argtypes = [C.POINTER(D2biRecord), C.POINTER(C.c_void_p), C.c_int32,
                                        _dp, _ip, C.c_int32, _dp, C.POINTER(D2biStats)]

@@ -92,9 +92,10 @@ struct AxisDev {
     const QL* ql[4];
 };
 
-static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
+// Copy the axis tables into workspace w.  t must stay alive until the
+// stream has executed the copies.
+static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s, Workspace& w)
 {
-    Workspace& w = g.ws;
     size_t nc = t.cell[0].size() + t.cell[1].size() + t.cell[2].size();
     size_t ns = t.start[0].size() + t.start[1].size() + t.start[2].size();
     size_t nq = nc + t.ql[3].size();
@@ -102,6 +103,31 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s)
     CHK(w.ax_zi.ensure(sizeof(double) * nc));
     CHK(w.ax_start.ensure(sizeof(int) * ns));
     CHK(w.ax_ql.ensure(sizeof(QL) * nq));
+    if (w.ax_cell.arena) {
+        // a workspace that must not allocate: copy from the tables
+        // themselves, not through the (growable) pinned staging buffer
+        size_t oc = 0, os = 0;
+        for (int a = 0; a < 3; a++) {
+            d->ql[a] = w.ax_ql.as<QL>() + oc;
+            d->cell[a] = w.ax_cell.as<int>() + oc;
+            d->zi[a] = w.ax_zi.as<double>() + oc;
+            d->start[a] = w.ax_start.as<int>() + os;
+            HIPCHK(hipMemcpyAsync((void*)d->ql[a], t.ql[a].data(), sizeof(QL) * t.ql[a].size(),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync((void*)d->cell[a], t.cell[a].data(), sizeof(int) * t.cell[a].size(),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync((void*)d->zi[a], t.zi[a].data(), sizeof(double) * t.zi[a].size(),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync((void*)d->start[a], t.start[a].data(), sizeof(int) * t.start[a].size(),
+                                  hipMemcpyHostToDevice, s));
+            oc += t.cell[a].size();
+            os += t.start[a].size();
+        }
+        d->ql[3] = w.ax_ql.as<QL>() + nc;
+        HIPCHK(hipMemcpyAsync((void*)d->ql[3], t.ql[3].data(), sizeof(QL) * t.ql[3].size(), hipMemcpyHostToDevice,
+                              s));
+        return D2BI_OK;
+    }
     size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns + sizeof(QL) * nq;
     void* hp;
     CHK(pinned(bytes, &hp));

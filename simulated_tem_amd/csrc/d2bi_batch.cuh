@@ -9,17 +9,6 @@ namespace d2bi {
 
 // ------------------------------------------------------------------ batched conditions
 
-__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-static void fill_i32(int* p, int64_t n, int v, hipStream_t s)
-{
-    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
-}
-
 // One condition of a batch: its plan, tables and slices of the batch buffers.
 struct Cond {
     d2bi_plan plan;
@@ -389,7 +378,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             int v = known_variant(&C[i].plan, C[i].dens);
             if (v < 0) {
                 d2bi_plan tmp;
-                CHK(image_condition(&recs[i], C[i].d_x, C[i].n, precAngle[i], nPrec[i], mode, nullptr, &tmp));
+                CHK(image_condition(&recs[i], C[i].d_x, C[i].n, precAngle[i], nPrec[i], mode, nullptr, &tmp,
+                                    g.ws));
                 v = known_variant(&C[i].plan, C[i].dens);
                 if (v < 0) v = 0;
             }

@@ -419,10 +419,10 @@ static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mp
                            w.slot_cs64.as<double2>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
 }
 
-// Run binning for either a condition (generic = 0) or plain points.
-static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, hipStream_t s)
+// Run binning for either a condition (generic = 0) or plain points, on
+// workspace w.
+static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, hipStream_t s, Workspace& w)
 {
-    Workspace& w = g.ws;
     const int64_t ncell64 = (int64_t)P.Ex * P.Ey * P.Ez;
     if (ncell64 > (int64_t)INT32_MAX / 2) return seterr(D2BI_ERR_CAPACITY, "too many link cells");
     const int ncell = (int)ncell64;
@@ -435,17 +435,17 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     CHK(w.cell_start.ensure(sizeof(int) * (ncell + 1)));
     CHK(w.blk_max.ensure(sizeof(int) * ((ncell + 255) / 256 + 1)));
     CHK(w.counters.ensure(64));
-    HIPCHK(hipMemsetAsync(w.cell_cnt.p, 0, sizeof(int) * (ncell + 1), s));
-    HIPCHK(hipMemsetAsync(w.counters.p, 0, 64, s));
+    fill_i32(w.cell_cnt.as<int>(), ncell + 1, 0, s);
+    fill_i32(w.counters.as<int>(), 16, 0, s);
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_classify, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
                            w.mask.as<uint32_t>(), w.blk_cnt.as<int>(), nblk);
-        HIPCHK(hipGetLastError());
     } else {
-        HIPCHK(hipMemsetAsync(w.blk_cnt.p, 0, sizeof(int) * ((int64_t)P.nuvw * nblk + 1), s));
+        fill_i32(w.blk_cnt.as<int>(), (int64_t)P.nuvw * nblk + 1, 0, s);
     }
+    HIPCHK(hipGetLastError());
     const int64_t nbc = (int64_t)P.nuvw * nblk;
-    CHK(scan_exclusive(w.blk_cnt.as<int>(), w.blk_off.as<int>(), nbc, s));
+    CHK(scan_exclusive(w.blk_cnt.as<int>(), w.blk_off.as<int>(), nbc, s, w));
     int nr_host = 0;
     HIPCHK(hipMemcpyAsync(&nr_host, w.blk_off.as<int>() + nbc, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -474,7 +474,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
                            w.pt_p32.as<float4>(), w.pt_sn32.as<float>(), w.cell_cnt.as<int>());
         HIPCHK(hipGetLastError());
     }
-    CHK(scan_exclusive(w.cell_cnt.as<int>(), w.cell_start.as<int>(), ncell, s));
+    CHK(scan_exclusive(w.cell_cnt.as<int>(), w.cell_start.as<int>(), ncell, s, w));
     if (nr > 0) {
         hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
                            w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.cell_start.as<int>(),

@@ -39,12 +39,32 @@ int seterr(int code, const char* fmt, ...)
 
 static uint64_t g_alloc_gen = 0;         // bumped on every device reallocation
 
+// A fixed block of device memory handed out front to back (256-byte
+// aligned pieces); buffers bound to an arena never call the allocator.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, off = 0;
+};
+
 struct Buf {
     void* p = nullptr;
     size_t cap = 0;
+    Arena* arena = nullptr;                  // set: pieces of the arena instead of hipMalloc
     int ensure(size_t bytes)
     {
         if (bytes <= cap && p) return D2BI_OK;
+        if (arena) {
+            const size_t off = (arena->off + 255) & ~(size_t)255;
+            if (off > arena->size || bytes > arena->size - off)
+                return seterr(D2BI_ERR_CAPACITY,
+                              "the no-allocation scratch (%zu bytes) cannot hold this condition: "
+                              "%zu bytes in use, %zu more requested",
+                              arena->size, arena->off, bytes);
+            p = arena->base + off;
+            cap = bytes;
+            arena->off = off + bytes;
+            return D2BI_OK;
+        }
         g_alloc_gen++;
         if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
         const size_t want = alloc_size(bytes);       // headroom for slightly larger conditions
@@ -58,7 +78,7 @@ struct Buf {
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p && !arena) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -94,7 +114,7 @@ struct Workspace {
     // fields and image
     Buf f64_re, f64_im, f64_rho, f32_field;
     Buf cnt, r2m;
-    Buf img;
+    Buf img, aa;                // image; 2 pi s_g per precession point (FP64 integration)
     Buf blur_in, blur_tmp, blur_out, blur_w, blur_wy, blur_k;
     // pinned host staging
     void* pinned = nullptr;
@@ -104,9 +124,14 @@ struct Workspace {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_p32, &pt_sn32,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
                       &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
-                      &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &blur_in, &blur_tmp,
+                      &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &aa, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_wy, &blur_k};
         for (Buf* b : all) f(*b);
+    }
+    // every buffer a piece of arena a (no device allocation from then on)
+    void bind(Arena* a)
+    {
+        each([&](Buf& b) { b.arena = a; });
     }
     // device bytes held
     size_t bytes()
@@ -174,6 +199,21 @@ struct d2bi_atoms_s {
 };
 
 namespace d2bi {
+
+// ------------------------------------------------------------------ fills
+
+// Fill kernels clear counters and flags (rather than hipMemsetAsync, which
+// must not be recorded into a captured graph).
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static void fill_i32(int* p, int64_t n, int v, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
 
 // ------------------------------------------------------------------ generic scan
 
@@ -277,10 +317,10 @@ static int scan_exclusive_tmp(const int* in, int* out, int64_t n, int* sums, hip
     return D2BI_OK;
 }
 
-static int scan_exclusive(const int* in, int* out, int64_t n, hipStream_t s)
+static int scan_exclusive(const int* in, int* out, int64_t n, hipStream_t s, Workspace& w)
 {
-    CHK(g.ws.scan_tmp.ensure(sizeof(int) * scan_tmp_ints(n)));
-    return scan_exclusive_tmp(in, out, n, g.ws.scan_tmp.as<int>(), s);
+    CHK(w.scan_tmp.ensure(sizeof(int) * scan_tmp_ints(n)));
+    return scan_exclusive_tmp(in, out, n, w.scan_tmp.as<int>(), s);
 }
 
 }  // namespace d2bi

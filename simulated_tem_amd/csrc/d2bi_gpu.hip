@@ -47,6 +47,30 @@
 
 using namespace d2bi;
 
+// ------------------------------------------------------------------ no-allocation scratch
+
+// The workspace of d2bi_gpu_image_noalloc: one static block in every device's
+// copy of the code object, held from module load to process end.  Host code
+// reaches a device's block only through the address hipGetSymbolAddress
+// returns with that device current (g_scratch_dev), never through the
+// symbol's name, which on the host is a stand-in and not a device address.
+static __device__ __align__(256) char scratch[D2BI_SCRATCH_BYTES];
+static constexpr int SCRATCH_MAX_DEVICES = 64;
+static void* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The current device's scratch address, looked up once per device.
+static int scratch_address(int device)
+{
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+        return seterr(D2BI_ERR_ARG, "device %d beyond the scratch table", device);
+    if (g_scratch_dev[device]) return D2BI_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return seterr(D2BI_ERR_HIP, "hipGetSymbolAddress gave no address for the scratch block");
+    g_scratch_dev[device] = p;
+    return D2BI_OK;
+}
+
 extern "C" {
 
 int d2bi_version(void) { return 10000; }
@@ -63,6 +87,10 @@ int d2bi_gpu_init(int device)
     if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return seterr(D2BI_ERR_NODEVICE, "no HIP device visible");
     if (device < 0 || device >= n) return seterr(D2BI_ERR_ARG, "device %d out of range (%d visible)", device, n);
     HIPCHK(hipSetDevice(device));
+    // the no-allocation scratch's address on this device; should the lookup
+    // fail, d2bi_gpu_image_noalloc repeats it and reports the error, and
+    // every other entry point works as before
+    if (scratch_address(device) != D2BI_OK) (void)hipGetLastError();
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     for (auto& e : g.ev) HIPCHK(hipEventCreate(&e));
     g.device = device;
@@ -128,7 +156,7 @@ int d2bi_gpu_image_dev(const d2bi_record* rec, d2bi_atoms atoms, double precAngl
     if (!rec || !atoms || !img) return seterr(D2BI_ERR_ARG, "null argument");
     if (stats) std::memset(stats, 0, sizeof *stats);
     d2bi_plan plan;
-    CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
+    CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan, g.ws));
     HIPCHK(hipMemcpy(img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToHost));
     return D2BI_OK;
 }
@@ -147,9 +175,51 @@ int d2bi_gpu_image_dev_out(const d2bi_record* rec, d2bi_atoms atoms, double prec
         return seterr(D2BI_ERR_ARG, "d_img is not memory of device %d", g.device);
     if (stats) std::memset(stats, 0, sizeof *stats);
     d2bi_plan plan;
-    CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan));
-    HIPCHK(hipMemcpyAsync(d_img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToDevice, g.stream));
+    CHK(image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan, g.ws, d_img));
     HIPCHK(hipStreamSynchronize(g.stream));
+    return D2BI_OK;
+}
+
+int d2bi_gpu_image_noalloc(const d2bi_record* rec, d2bi_atoms atoms, double precAngle, int32_t nPrec,
+                           int32_t mode, double* d_img, d2bi_stats* stats)
+{
+    CHK(need_init());
+    if (!rec || !atoms || !d_img) return seterr(D2BI_ERR_ARG, "null argument");
+    if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
+    CHK(scratch_address(g.device));                 // done at init unless it failed there
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, d_img) != hipSuccess) {
+        (void)hipGetLastError();
+        return seterr(D2BI_ERR_ARG, "d_img is not device memory");
+    }
+    if (attr.type != hipMemoryTypeDevice || attr.device != g.device)
+        return seterr(D2BI_ERR_ARG, "d_img is not memory of device %d", g.device);
+    // refuse, before anything is launched, a condition whose field, link
+    // cells and replica masks alone exceed the scratch; the arena refuses
+    // the rest (point arrays) when the point count is known
+    d2bi_plan plan;
+    CHK(plan_image(rec, atoms->n, precAngle, nPrec, &plan));
+    const double nvox = (double)plan.Mx * plan.My * plan.Mz;
+    const double ncell = (double)plan.Ex * plan.Ey * plan.Ez;
+    const double need = nvox * (mode == D2BI_MODE_FP64 ? 24.0 : 16.0) + 8.0 * (ncell + 1) +
+                        4.0 * (double)atoms->n;
+    if (need > (double)D2BI_SCRATCH_BYTES)
+        return seterr(D2BI_ERR_CAPACITY,
+                      "condition of %d x %d x %d voxels needs at least %.0f bytes of workspace; the "
+                      "no-allocation scratch holds %lld",
+                      plan.Mx, plan.My, plan.Mz, need, (long long)D2BI_SCRATCH_BYTES);
+    Arena ar;
+    ar.base = static_cast<char*>(g_scratch_dev[g.device]);
+    ar.size = (size_t)D2BI_SCRATCH_BYTES;
+    Workspace w;
+    w.bind(&ar);
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    const int rc = image_condition(rec, atoms->x, atoms->n, precAngle, nPrec, mode, stats, &plan, w, d_img);
+    // work already queued on the scratch finishes before the call returns,
+    // also after an error
+    const hipError_t e = hipStreamSynchronize(g.stream);
+    if (rc != D2BI_OK) return rc;
+    if (e != hipSuccess) return seterr(D2BI_ERR_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
     return D2BI_OK;
 }
 
@@ -178,7 +248,7 @@ int d2bi_gpu_image(const d2bi_record* rec, const double* x, int64_t ld, int64_t 
     double* d_x;
     CHK(upload_atoms_tmp(x, ld, nAtoms, &d_x));
     d2bi_plan plan;
-    CHK(image_condition(rec, d_x, nAtoms, precAngle, nPrec, mode, stats, &plan));
+    CHK(image_condition(rec, d_x, nAtoms, precAngle, nPrec, mode, stats, &plan, g.ws));
     HIPCHK(hipMemcpy(img, g.ws.img.p, sizeof(double) * plan.Mx * plan.My, hipMemcpyDeviceToHost));
     return D2BI_OK;
 }
@@ -234,13 +304,13 @@ int d2bi_gpu_field(const d2bi_record* rec, const double* x, int64_t ld, int64_t 
     BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
     hipStream_t s = g.stream;
     AxisDev ax;
-    CHK(upload_axis(tab, &ax, s));
+    CHK(upload_axis(tab, &ax, s, g.ws));
     Binned b;
-    CHK(run_binning(d_x, nAtoms, BP, &b, s));
+    CHK(run_binning(d_x, nAtoms, BP, &b, s, g.ws));
     // FAST without count and r2min runs the imaging kernel itself (the
     // variant imaging uses)
     const bool diag = mode == D2BI_MODE_FP64 || count || r2min;
-    CHK(run_voxel(rec, &plan, tab, ax, mode, diag, nullptr, s));
+    CHK(run_voxel(rec, &plan, tab, ax, mode, diag, nullptr, s, g.ws));
     HIPCHK(hipStreamSynchronize(s));
     const int64_t nvox = (int64_t)plan.Mx * plan.My * plan.Mz;
     Workspace& w = g.ws;
@@ -285,7 +355,7 @@ int d2bi_gpu_linkcells(const d2bi_record* rec, const double* x, int64_t ld, int6
     BinParams BP;
     CHK(bin_params(rec, &plan, &BP));
     Binned b;
-    CHK(run_binning(d_x, nAtoms, BP, &b, g.stream));
+    CHK(run_binning(d_x, nAtoms, BP, &b, g.stream, g.ws));
     HIPCHK(hipStreamSynchronize(g.stream));
     if (nPoints) *nPoints = b.nr;
     if ((ids || frac || gdotr) && b.nr > cap)
@@ -322,7 +392,7 @@ int d2bi_gpu_lc_query(const double a[9], int32_t Nx, int32_t Ny, int32_t Nz, con
     BP.generic = 1;
     BP.cell_a[0] = a[0]; BP.cell_a[1] = a[4]; BP.cell_a[2] = a[8];
     Binned b;
-    CHK(run_binning(d_x, nP, BP, &b, s));
+    CHK(run_binning(d_x, nP, BP, &b, s, g.ws));
     QueryParams Q;
     std::memcpy(Q.a, a, sizeof Q.a);
     std::memcpy(Q.ia, ia, sizeof Q.ia);

@@ -201,11 +201,11 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, T
     return best;
 }
 
-// Voxelise into the mode's field buffers; optional count / r2min outputs.
+// Voxelise into the mode's field buffers of workspace w; optional count /
+// r2min outputs.
 static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables& tab, const AxisDev& ax,
-                     int mode, bool want_diag, int64_t* n_guard, hipStream_t s, Timer* T = nullptr)
+                     int mode, bool want_diag, int64_t* n_guard, hipStream_t s, Workspace& w, Timer* T = nullptr)
 {
-    Workspace& w = g.ws;
     const int64_t nvox = (int64_t)p->Mx * p->My * p->Mz;
     VoxParams V;
     fill_vox_params(r, p, ax, &V);
@@ -262,7 +262,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         return ms;
     });
     if (variant < 0) return variant;
-    HIPCHK(hipMemsetAsync(gp, 0, gp_bytes, s));     // tuning launches (D2BI_FAST_TUNE=1) counted too
+    fill_i32((int*)gp, (int64_t)(gp_bytes / sizeof(int)), 0, s);   // tuning launches (D2BI_FAST_TUNE=1) counted too
     if (T) CHK(T->record(2));
     CHK(run_variant(variant));
     HIPCHK(hipGetLastError());
@@ -352,17 +352,16 @@ static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double*
     return D2BI_OK;
 }
 
-static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s, Timer* T = nullptr)
+// Integrate the field of workspace w into d_img (Mx*My doubles).
+static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s, Workspace& w, double* d_img,
+                         Timer* T = nullptr)
 {
-    Workspace& w = g.ws;
-    const int64_t plane = (int64_t)p->Mx * p->My;
-    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
-    double* d_img = w.img.as<double>();
     if (mode == D2BI_MODE_FP64) {
-        double* d_aa = d_img + plane;
+        CHK(w.aa.ensure(sizeof(double) * D2BI_MAX_PREC));
+        double* d_aa = w.aa.as<double>();
         HIPCHK(hipMemcpyAsync(d_aa, p->aa, sizeof(double) * p->nPrec, hipMemcpyHostToDevice, s));
         if (T) CHK(T->record(4));
-        CHK(launch_integrate_fp64(p, d_aa, d_img, s));
+        CHK(launch_integrate_fp64(p, d_aa, d_img, s, 0, 0.0, &w));
         if (T) CHK(T->record(5));
         // the aa copy reads p->aa (caller memory): finish before returning
         HIPCHK(hipStreamSynchronize(s));
@@ -374,9 +373,11 @@ static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s,
     return D2BI_OK;
 }
 
-// Whole condition: bin -> voxelise -> integrate, result left in ws.img.
+// Whole condition: bin -> voxelise -> integrate on workspace w, the image
+// written to d_img (device memory of Mx*My doubles) or, when d_img is null,
+// left in w.img.
 static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
-                           int mode, d2bi_stats* st, d2bi_plan* plan_out)
+                           int mode, d2bi_stats* st, d2bi_plan* plan_out, Workspace& w, double* d_img = nullptr)
 {
     if (mode != D2BI_MODE_FP64 && mode != D2BI_MODE_FAST) return seterr(D2BI_ERR_ARG, "unknown mode %d", mode);
     d2bi_plan plan;
@@ -386,26 +387,25 @@ static int image_condition(const d2bi_record* r, const double* d_x, int64_t n, d
     BinParams BP;
     CHK(bin_params(r, &plan, &BP));
     BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
+    if (!d_img) {
+        CHK(w.img.ensure(sizeof(double) * (int64_t)plan.Mx * plan.My));
+        d_img = w.img.as<double>();
+    }
     hipStream_t s = g.stream;
     Timer T{g.ev, s};
     CHK(T.record(0));
     AxisDev ax;
-    CHK(upload_axis(tab, &ax, s));
+    CHK(upload_axis(tab, &ax, s, w));
     Binned b;
-    CHK(run_binning(d_x, n, BP, &b, s));
+    CHK(run_binning(d_x, n, BP, &b, s, w));
     CHK(T.record(1));
-    CHK(run_voxel(r, &plan, tab, ax, mode, false, nullptr, s, &T));
-    CHK(run_integrate(&plan, r->use_density != 0, mode, s, &T));
+    CHK(run_voxel(r, &plan, tab, ax, mode, false, nullptr, s, w, &T));
+    CHK(run_integrate(&plan, r->use_density != 0, mode, s, w, d_img, &T));
     CHK(T.record(6));
     // counters (largest cell count at 0, FAST guard re-tests at 8) come back
     // with the final synchronisation
-    int64_t* hc = nullptr;
-    if (st) {
-        void* hp;
-        CHK(pinned(16, &hp));
-        hc = (int64_t*)hp;
-        HIPCHK(hipMemcpyAsync(hc, g.ws.counters.p, 16, hipMemcpyDeviceToHost, s));
-    }
+    int64_t hc[2] = {0, 0};
+    if (st) HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (st) {
         const int mpc = (int)(hc[0] & 0xffffffff);
@@ -448,11 +448,12 @@ static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, do
     AxisTables tab;
     CHK(axis_tables(r, &plan, &tab));
     hipStream_t s = g.stream;
-    AxisDev ax;
-    CHK(upload_axis(tab, &ax, s));
     Workspace& w = g.ws;
+    AxisDev ax;
+    CHK(upload_axis(tab, &ax, s, w));
     const int64_t plane = (int64_t)plan.Mx * plan.My;
-    CHK(w.img.ensure(sizeof(double) * plane + sizeof(double) * D2BI_MAX_PREC));
+    CHK(w.img.ensure(sizeof(double) * plane));
+    CHK(w.aa.ensure(sizeof(double) * D2BI_MAX_PREC));
     double* d_img = w.img.as<double>();
     const bool dens = r->use_density != 0;
     const double total = (double)nApp * nPrec;
@@ -464,23 +465,18 @@ static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, do
         CHK(bin_params(r, &pk, &BP));
         BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
         Binned b;
-        CHK(run_binning(d_x, n, BP, &b, s));
-        CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s));
+        CHK(run_binning(d_x, n, BP, &b, s, w));
+        CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s, w));
         if (mode == D2BI_MODE_FP64) {
-            double* d_aa = d_img + plane;
+            double* d_aa = w.aa.as<double>();
             HIPCHK(hipMemcpyAsync(d_aa, pk.aa, sizeof(double) * nPrec, hipMemcpyHostToDevice, s));
             // app 0 writes its sum (div 1), the others add theirs
-            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0));
+            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0, &w));
         } else {
             CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total));
         }
-        int64_t* hc = nullptr;
-        if (st) {
-            void* hp;
-            CHK(pinned(16, &hp));
-            hc = (int64_t*)hp;
-            HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
-        }
+        int64_t hc[2] = {0, 0};
+        if (st) HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
         // pk.aa (host) is read by the copy above: finish before the next point
         HIPCHK(hipStreamSynchronize(s));
         if (st) {
