@@ -139,6 +139,10 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, Workspace& w
     FastParams F;
     int variant = c.variant;
     CHK(variant_params(p, c.tab, V, &variant, &F));
+    // the tile order timed eagerly before capture (nothing is timed here)
+    static const int ocand[2] = {0, 1};
+    F.order = tune_choose(tile_order_key(p, c.tab, c.dens, c.variant), "D2BI_TILE_ORDER", ocand, 2, 0, false, s,
+                          [](int) {});
     launch_tile<false>(F, variant, c.dens, w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
                        gp, s);
@@ -358,25 +362,13 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             if (!B.stream[k - 1]) HIPCHK(hipStreamCreateWithFlags(&B.stream[k - 1], hipStreamNonBlocking));
             if (!B.join[k - 1]) HIPCHK(hipEventCreateWithFlags(&B.join[k - 1], hipEventDisableTiming));
         }
-        // conditions to branches: largest first, each to the least loaded
-        // branch (a fixed rule: the same inputs give the same graph)
-        std::vector<int> order(nRec), branch(nRec, 0);
-        for (int i = 0; i < nRec; i++) order[i] = i;
-        auto work = [&](int i) { return (double)C[i].nvox * (1.0 + C[i].plan.nPrec / 8.0) + 8.0 * C[i].n; };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
-        std::vector<double> load(nbr, 0.0);
-        for (int i : order) {
-            int k = 0;
-            for (int j = 1; j < nbr; j++)
-                if (load[j] < load[k]) k = j;
-            branch[i] = k;
-            load[k] += work(i);
-        }
         // FAST kernel variants (fixed, forced, or with D2BI_FAST_TUNE=1 tuned
-        // eagerly for unseen shapes, outside the graph)
+        // eagerly for unseen shapes, outside the graph), and the launch
+        // choices of shapes the tuner has not timed yet: one eager condition
+        // of each such shape times them (nothing is timed inside the graph)
         for (int i = 0; i < nRec && mode == D2BI_MODE_FAST; i++) {
             int v = known_variant(&C[i].plan, C[i].dens);
-            if (v < 0) {
+            if (v < 0 || !condition_tuned(&C[i].plan, C[i].tab, C[i].dens, v)) {
                 d2bi_plan tmp;
                 CHK(image_condition(&recs[i], C[i].d_x, C[i].n, precAngle[i], nPrec[i], mode, nullptr, &tmp,
                                     g.ws));
@@ -385,39 +377,101 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             }
             C[i].variant = v;
         }
-        // eager tuning may have grown workspace buffers: capture after it
-        hipGraph_t graph;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = D2BI_OK;
-        auto bstream = [&](int k) { return k == 0 ? s : B.stream[k - 1]; };
-        if (nbr > 1) {
-            if (hipEventRecord(B.fork, s) != hipSuccess) rc = seterr(D2BI_ERR_HIP, "graph fork failed");
-            for (int k = 1; k < nbr && rc == D2BI_OK; k++)
-                if (hipStreamWaitEvent(bstream(k), B.fork, 0) != hipSuccess) rc = seterr(D2BI_ERR_HIP, "graph fork failed");
+        // capture the batch on nb branches: conditions to branches largest
+        // first, each to the least loaded branch (a fixed rule: the same
+        // inputs give the same graph); eager tuning may have grown workspace
+        // buffers, so this comes after it
+        auto capture = [&](int nb, hipGraphExec_t* exec) -> int {
+            std::vector<int> order(nRec), branch(nRec, 0);
+            for (int i = 0; i < nRec; i++) order[i] = i;
+            auto work = [&](int i) { return (double)C[i].nvox * (1.0 + C[i].plan.nPrec / 8.0) + 8.0 * C[i].n; };
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+            std::vector<double> load(nb, 0.0);
+            for (int i : order) {
+                int k = 0;
+                for (int j = 1; j < nb; j++)
+                    if (load[j] < load[k]) k = j;
+                branch[i] = k;
+                load[k] += work(i);
+            }
+            hipGraph_t graph;
+            g_capturing = true;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                g_capturing = false;
+                return seterr(D2BI_ERR_HIP, "graph capture failed to begin");
+            }
+            int rc = D2BI_OK;
+            auto bstream = [&](int k) { return k == 0 ? s : B.stream[k - 1]; };
+            if (nb > 1) {
+                if (hipEventRecord(B.fork, s) != hipSuccess) rc = seterr(D2BI_ERR_HIP, "graph fork failed");
+                for (int k = 1; k < nb && rc == D2BI_OK; k++)
+                    if (hipStreamWaitEvent(bstream(k), B.fork, 0) != hipSuccess)
+                        rc = seterr(D2BI_ERR_HIP, "graph fork failed");
+            }
+            for (int i = 0; i < nRec && rc == D2BI_OK; i++) {
+                Workspace& w = B.branch_ws(branch[i]);
+                rc = enqueue_count(C[i], B, w.scan_tmp.as<int>(), bstream(branch[i]));
+                if (rc == D2BI_OK) rc = enqueue_rest(C[i], i, B, mode, w, bstream(branch[i]));
+            }
+            for (int k = 1; k < nb; k++) {
+                // join every branch back, also after an error, so the capture can end
+                if (hipEventRecord(B.join[k - 1], bstream(k)) != hipSuccess ||
+                    hipStreamWaitEvent(s, B.join[k - 1], 0) != hipSuccess)
+                    if (rc == D2BI_OK) rc = seterr(D2BI_ERR_HIP, "graph join failed");
+            }
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            g_capturing = false;
+            if (rc != D2BI_OK) {
+                if (e == hipSuccess) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return seterr(D2BI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                *exec = nullptr;
+                return seterr(D2BI_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+            }
+            return D2BI_OK;
+        };
+        // branch count: forced, cached, or tuned by timing one run of each
+        // count's graph (images are the same on any count)
+        int chosen = nbr;
+        if (nbr > 1 && !std::getenv("D2BI_BATCH_BRANCHES") && tuning_on()) {
+            int64_t sum_vox = 0, max_vox_c = 0, sum_n = 0, sum_p = 0;
+            for (int i = 0; i < nRec; i++) {
+                sum_vox += C[i].nvox;
+                max_vox_c = std::max(max_vox_c, C[i].nvox);
+                sum_n += C[i].n;
+                sum_p += C[i].plan.nPrec;
+            }
+            const TuneKey bkey{TUNE_BRANCHES, nRec, mode, sum_vox, max_vox_c, sum_n, sum_p, nbr, 0};
+            const int c = tune_get(bkey);
+            if (c >= 1 && c <= nbr) {
+                chosen = c;
+            } else {
+                float tbest = 3.4e38f;
+                for (int nb = 1; nb <= nbr; nb++) {
+                    hipGraphExec_t ex = nullptr;
+                    CHK(capture(nb, &ex));
+                    (void)hipGraphUpload(ex, s);
+                    const float t = time_launch([&] { (void)hipGraphLaunch(ex, s); }, s, 1);
+                    if (t >= 0.0f && t < tbest) {
+                        tbest = t;
+                        chosen = nb;
+                        if (B.exec) (void)hipGraphExecDestroy(B.exec);
+                        B.exec = ex;
+                    } else {
+                        (void)hipGraphExecDestroy(ex);
+                    }
+                }
+                tune_put(bkey, chosen);
+            }
         }
-        for (int i = 0; i < nRec && rc == D2BI_OK; i++) {
-            Workspace& w = B.branch_ws(branch[i]);
-            rc = enqueue_count(C[i], B, w.scan_tmp.as<int>(), bstream(branch[i]));
-            if (rc == D2BI_OK) rc = enqueue_rest(C[i], i, B, mode, w, bstream(branch[i]));
-        }
-        for (int k = 1; k < nbr; k++) {
-            // join every branch back, also after an error, so the capture can end
-            if (hipEventRecord(B.join[k - 1], bstream(k)) != hipSuccess ||
-                hipStreamWaitEvent(s, B.join[k - 1], 0) != hipSuccess)
-                if (rc == D2BI_OK) rc = seterr(D2BI_ERR_HIP, "graph join failed");
-        }
-        hipError_t e = hipStreamEndCapture(s, &graph);
-        if (rc != D2BI_OK) {
-            if (e == hipSuccess) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        if (e != hipSuccess) return seterr(D2BI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            B.exec = nullptr;
-            return seterr(D2BI_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
-        }
+        if (!B.exec) CHK(capture(chosen, &B.exec));
+        // workspaces of branches the graph does not use go back to the device
+        for (int k = std::max(1, chosen); k < nbr; k++) B.branch_ws(k).release_all();
+        B.nbr = chosen;
         B.key = key;
         B.gen = g_alloc_gen;
         B.nr.resize(nRec);

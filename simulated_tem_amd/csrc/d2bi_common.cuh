@@ -115,6 +115,7 @@ struct Workspace {
     Buf f64_re, f64_im, f64_rho, f32_field;
     Buf cnt, r2m;
     Buf img, aa;                // image; 2 pi s_g per precession point (FP64 integration)
+    Buf tune_img;               // image written by timing launches of the tuner
     Buf blur_in, blur_tmp, blur_out, blur_w, blur_wy, blur_k;
     // pinned host staging
     void* pinned = nullptr;
@@ -124,7 +125,7 @@ struct Workspace {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_p32, &pt_sn32,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
                       &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
-                      &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &aa, &blur_in, &blur_tmp,
+                      &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &aa, &tune_img, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_wy, &blur_k};
         for (Buf* b : all) f(*b);
     }

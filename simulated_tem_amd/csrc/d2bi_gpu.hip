@@ -39,6 +39,7 @@
 #include "d2bi_fp64.cuh"
 #include "d2bi_voxel_tile.cuh"
 #include "d2bi_integrate_fast.cuh"
+#include "d2bi_tune.cuh"
 #include "d2bi_blur_query.cuh"
 #include "d2bi_run.cuh"
 #include "d2bi_batch.cuh"

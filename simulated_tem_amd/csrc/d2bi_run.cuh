@@ -66,6 +66,7 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     F.ntz = (p->Mz + tz - 1) / tz;
     F.ntiles = F.ntx * F.nty * F.ntz;
     F.per_xcd = (F.ntiles + 7) / 8;
+    F.order = 0;
     // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
     // rounded once from FP64, three FMAs); +-1e-5 relative covers it 5x
     const double k2 = -p->i2s2 * 1.4426950408889634;
@@ -201,6 +202,14 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, T
     return best;
 }
 
+// Tuner key of the FAST voxel kernel's tile order: the shape and the variant
+// that runs (after the fallback of variant_params).
+static TuneKey tile_order_key(const d2bi_plan* p, const AxisTables& tab, bool dens, int variant)
+{
+    const int v = (variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) ? 1 : variant;
+    return TuneKey{TUNE_TILE_ORDER, p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0, v};
+}
+
 // Voxelise into the mode's field buffers of workspace w; optional count /
 // r2min outputs.
 static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables& tab, const AxisDev& ax,
@@ -239,10 +248,12 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     CHK(w.guard_part.ensure(gp_bytes));
     unsigned long long* gp = w.guard_part.as<unsigned long long>();
     const bool dens = r->use_density != 0;
-    // launch parameters follow the variant's tile depth
-    auto run_variant = [&](int v) -> int {
+    // launch parameters follow the variant's tile depth; order: the tiles'
+    // order over the workgroups (0 per XCD, 1 linear)
+    auto run_variant = [&](int v, int order = 0) -> int {
         FastParams Fv;
         CHK(variant_params(p, tab, V, &v, &Fv));
+        Fv.order = order;
         pick_tile(want_diag, dens, v, Fv.thin_z != 0, [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(8 * Fv.per_xcd), dim3(64), 0, s, Fv, w.cell_start.as<int>(),
                                w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
@@ -262,9 +273,17 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         return ms;
     });
     if (variant < 0) return variant;
-    fill_i32((int*)gp, (int64_t)(gp_bytes / sizeof(int)), 0, s);   // tuning launches (D2BI_FAST_TUNE=1) counted too
+    // the tile order: timed the first time a shape is seen (each launch
+    // overwrites the whole field, so the timing runs leave no trace)
+    int order = 0;
+    if (!want_diag) {
+        static const int ocand[2] = {0, 1};
+        order = tune_choose(tile_order_key(p, tab, dens, variant), "D2BI_TILE_ORDER", ocand, 2, 0, true, s,
+                            [&](int o) { (void)run_variant(variant, o); });
+    }
+    fill_i32((int*)gp, (int64_t)(gp_bytes / sizeof(int)), 0, s);   // tuning launches counted too
     if (T) CHK(T->record(2));
-    CHK(run_variant(variant));
+    CHK(run_variant(variant, order));
     HIPCHK(hipGetLastError());
     if (T) CHK(T->record(3));
     hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
@@ -286,35 +305,78 @@ static bool fast_integrator_poly()
     return !(e && std::strcmp(e, "stage") == 0);
 }
 
+// Points of the next FAST integration pass when `left` remain.
+static int pass_points(int left)
+{
+    return left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+}
+
+// Tuner key of the FAST integrators' workgroup size for a pass of np points.
+static TuneKey int_block_key(const d2bi_plan* p, int np, bool dens)
+{
+    return TuneKey{TUNE_INT_BLOCK, p->Mx, p->My, p->Mz, np, dens ? 1 : 0, fast_integrator_poly() ? 1 : 0, 0, 0};
+}
+
+// Whether every tuned launch choice of a FAST condition is forced, already
+// timed, or not tuned at all.
+static bool condition_tuned(const d2bi_plan* p, const AxisTables& tab, bool dens, int variant)
+{
+    if (!tuning_on()) return true;
+    if (forced_int("D2BI_TILE_ORDER", 0, 1) < 0 && tune_get(tile_order_key(p, tab, dens, variant)) < 0) return false;
+    if (forced_int("D2BI_INT_BLOCK", 64, 256) >= 0) return true;
+    for (int p0 = 0; p0 < p->nPrec;) {
+        const int np = pass_points(p->nPrec - p0);
+        if (tune_get(int_block_key(p, np, dens)) < 0) return false;
+        p0 += np;
+    }
+    return true;
+}
+
+// One pass of the FAST integrators over NP points, workgroups of `block`
+// threads (a lane per pixel column; any block size gives the same image).
 template <int NP>
 static void launch_int_fast(const FastIntParams& P, bool dens, const void* field, double* img, int acc,
-                            double inv, hipStream_t s)
+                            double inv, int block, hipStream_t s)
 {
     const int64_t plane = (int64_t)P.Mx * P.My;
-    const dim3 grid((unsigned)((plane + 255) / 256));
+    const dim3 grid((unsigned)((plane + block - 1) / block)), thr(block);
     // from 8 points on, the shared step matrix costs less than it saves
     if constexpr (NP >= 8) {
         if (fast_integrator_poly()) {
-            if (dens) hipLaunchKernelGGL((k_integrate_poly<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
-            else hipLaunchKernelGGL((k_integrate_poly<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+            if (dens) hipLaunchKernelGGL((k_integrate_poly<NP, true>), grid, thr, 0, s, P, field, img, acc, inv);
+            else hipLaunchKernelGGL((k_integrate_poly<NP, false>), grid, thr, 0, s, P, field, img, acc, inv);
             return;
         }
     }
-    if (dens) hipLaunchKernelGGL((k_integrate_fast<NP, true>), grid, dim3(256), 0, s, P, field, img, acc, inv);
-    else hipLaunchKernelGGL((k_integrate_fast<NP, false>), grid, dim3(256), 0, s, P, field, img, acc, inv);
+    if (dens) hipLaunchKernelGGL((k_integrate_fast<NP, true>), grid, thr, 0, s, P, field, img, acc, inv);
+    else hipLaunchKernelGGL((k_integrate_fast<NP, false>), grid, thr, 0, s, P, field, img, acc, inv);
+}
+
+static void launch_int_pass(int np, const FastIntParams& P, bool dens, const void* f, double* img, int acc,
+                            double inv, int block, hipStream_t s)
+{
+    switch (np) {
+        case 32: launch_int_fast<32>(P, dens, f, img, acc, inv, block, s); break;
+        case 16: launch_int_fast<16>(P, dens, f, img, acc, inv, block, s); break;
+        case 8: launch_int_fast<8>(P, dens, f, img, acc, inv, block, s); break;
+        case 4: launch_int_fast<4>(P, dens, f, img, acc, inv, block, s); break;
+        case 2: launch_int_fast<2>(P, dens, f, img, acc, inv, block, s); break;
+        default: launch_int_fast<1>(P, dens, f, img, acc, inv, block, s); break;
+    }
 }
 
 // FAST integration of all precession points, in passes of up to 32.
 // dens: the field holds float4 {x, rho} (density field on), else float2 x.
 // acc_in: add to d_img from the first pass on; inv: weight of one
-// integration (0: 1/nPrec).
+// integration (0: 1/nPrec).  tune_ws: a workspace whose tune_img buffer may
+// grow, for timing the workgroup sizes of a pass that adds to d_img (null:
+// such a pass is not timed).
 static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, double* d_img, hipStream_t s,
-                                 bool acc_in = false, double inv = 0.0)
+                                 bool acc_in = false, double inv = 0.0, Workspace* tune_ws = nullptr)
 {
     if (inv == 0.0) inv = 1.0 / p->nPrec;
     for (int p0 = 0, first = acc_in ? 0 : 1; p0 < p->nPrec; first = 0) {
-        const int left = p->nPrec - p0;
-        const int np = left >= 32 ? 32 : left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+        const int np = pass_points(p->nPrec - p0);
         FastIntParams P;
         P.Mx = p->Mx; P.My = p->My; P.Mz = p->Mz; P.np = np;
         P.b0 = (float)p->b0; P.bg = (float)p->bg; P.dz = (float)p->dz;
@@ -322,14 +384,19 @@ static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, d
             P.aa[i] = i < np ? (float)p->aa[p0 + i] : 0.0f;
             P.al[i] = i < np ? (float)(2.0 * p->dz * p->aa[p0 + i]) : 0.0f;
         }
-        switch (np) {
-            case 32: launch_int_fast<32>(P, dens, f, d_img, !first, inv, s); break;
-            case 16: launch_int_fast<16>(P, dens, f, d_img, !first, inv, s); break;
-            case 8: launch_int_fast<8>(P, dens, f, d_img, !first, inv, s); break;
-            case 4: launch_int_fast<4>(P, dens, f, d_img, !first, inv, s); break;
-            case 2: launch_int_fast<2>(P, dens, f, d_img, !first, inv, s); break;
-            default: launch_int_fast<1>(P, dens, f, d_img, !first, inv, s); break;
+        // workgroup size: timed on a launch that writes (not adds to) an
+        // image, the output itself for the first pass, else a tuning buffer
+        double* timg = first ? d_img : nullptr;
+        const TuneKey key = int_block_key(p, np, dens);
+        if (!timg && tune_ws && !tune_ws->tune_img.arena && !g_capturing && tuning_on() && tune_get(key) < 0) {
+            CHK(tune_ws->tune_img.ensure(sizeof(double) * (int64_t)P.Mx * P.My));
+            timg = tune_ws->tune_img.as<double>();
         }
+        static const int cand[3] = {64, 128, 256};
+        const int block = tune_choose(key, "D2BI_INT_BLOCK", cand, 3, 256, timg != nullptr, s, [&](int b) {
+            launch_int_pass(np, P, dens, f, timg, 0, inv, b, s);
+        });
+        launch_int_pass(np, P, dens, f, d_img, !first, inv, block, s);
         HIPCHK(hipGetLastError());
         p0 += np;
     }
@@ -368,7 +435,7 @@ static int run_integrate(const d2bi_plan* p, bool dens, int mode, hipStream_t s,
         return D2BI_OK;
     }
     if (T) CHK(T->record(4));
-    CHK(launch_integrate_fast(p, dens, w.f32_field.p, d_img, s));
+    CHK(launch_integrate_fast(p, dens, w.f32_field.p, d_img, s, false, 0.0, &w));
     if (T) CHK(T->record(5));
     return D2BI_OK;
 }
@@ -473,7 +540,7 @@ static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, do
             // app 0 writes its sum (div 1), the others add theirs
             CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0, &w));
         } else {
-            CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total));
+            CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total, &w));
         }
         int64_t hc[2] = {0, 0};
         if (st) HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));

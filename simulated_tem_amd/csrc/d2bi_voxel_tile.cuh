@@ -14,6 +14,7 @@ static constexpr int TILE_CAP = 160;                 // staged candidates per pa
 static constexpr int GUARD_SLOTS = 1024;             // guard counters (power of two) ...
 static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart
 static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
+static constexpr int ROW_BLOCK = 16;                 // row kernel: steps per guard test and Kahan merge
 
 // Inclusive prefix sum over the 64 lanes of a wave in six DPP steps: within
 // each row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill), then row 0's
@@ -43,6 +44,7 @@ struct FastParams {
     float cellf[3];             // link-cell sides in FP32
     int thin_z;                 // link cells thinner than 3 sigma along z
     int tz;                     // tile depth in voxels (4, or 8 for the 4 x 4 x 8 row kernel)
+    int order;                  // tiles over workgroups: 0 contiguous ranges per XCD, 1 linear
 };
 
 // One wave per voxel tile: 4x4x4 voxels, or 4x4x8 for the row kernel with
@@ -120,10 +122,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     constexpr int NQ = ROW ? (TZ == 8 ? 2 : 4) : 1;
     constexpr int NV = ROW ? 4 / NQ : 1;
     // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
-    // give each XCD a contiguous range of tiles so neighbouring tiles, which
-    // read the same link cells, share one L2.
+    // order 0 gives each XCD a contiguous range of tiles so neighbouring
+    // tiles, which read the same link cells, share one L2; order 1 is linear.
+    // A tile's sums do not depend on the order.
     const int bid = blockIdx.x;
-    const int L = (bid & 7) * F.per_xcd + (bid >> 3);
+    const int L = F.order ? bid : (bid & 7) * F.per_xcd + (bid >> 3);
     if (L >= F.ntiles) return;
     const VoxParams& P = F.v;
     const int tx = L % F.ntx, ty = (L / F.ntx) % F.nty, tz = L / (F.ntx * F.nty);
@@ -270,9 +273,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     float emax = -3.4028234663852886e38f;
     int cnt = 0;
     unsigned nguard = 0;
-    // ROW: sums and largest u of the row's four voxels over this lane's part
-    // of the candidates
+    // ROW: sums (with their Kahan compensations) and largest u of the row's
+    // four voxels over this lane's part of the candidates
     float rre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float kre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, kim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float umax[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
 
     int t_next = 0, kbase = 0;
@@ -405,18 +409,22 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 kahan_add(sim, cim, pi);
             }
         } else {
-            // ROW: steps of NQ candidates (one per part), blocks of 8 steps;
-            // the guard band in u is the exponent's band (|u| ~ ln2 |t| near
-            // the threshold, so it is 1.44 times as wide).  The member weight
-            // is X E = u + 1, taken as w = s (u + 1) with the step
+            // ROW: steps of NQ candidates (one per part), blocks of ROW_BLOCK
+            // steps; the guard band in u is the exponent's band (|u| ~ ln2 |t|
+            // near the threshold, so it is 1.44 times as wide).  The member
+            // weight is X E = u + 1, taken as w = s (u + 1) with the step
             // s = clamp(2^40 u) in [0, 1]: 1 for u >= 2^-40, 0 for u <= 0, and
             // fractional only inside the guard band, whose pairs the block's
-            // re-walk settles.
+            // re-walk settles.  Each block's sums are added to the running
+            // sums with Kahan compensation, and so are the re-walk's
+            // corrections (the block sums are not live across the re-walk,
+            // which sets the register peak).
             const int nst = (n + NQ - 1) / NQ;
             auto step = [](float u) { return __builtin_amdgcn_fmed3f(u * 1099511627776.0f, 0.0f, 1.0f); };
-            for (int kb = 0; kb < nst; kb += 8) {
-                const int ke = min(kb + 8, nst);
+            for (int kb = 0; kb < nst; kb += ROW_BLOCK) {
+                const int ke = min(kb + ROW_BLOCK, nst);
                 float dmin = 3.4028234663852886e38f;
+                float bre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, bim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
                 for (int kk = kb; kk < ke; kk++) {
                     const int j = NQ * kk + squart;
@@ -437,11 +445,16 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                         const float u = fmaf(xv[i], E, -1.0f);
                         const float s = step(u);
                         const float w = fmaf(s, u, s);
-                        rre[i] = fmaf(w, B.w, rre[i]);
-                        rim[i] = fmaf(w, sn, rim[i]);
+                        bre[i] = fmaf(w, B.w, bre[i]);
+                        bim[i] = fmaf(w, sn, bim[i]);
                         dmin = fminf(dmin, fabsf(u));
                         if (DENSITY) umax[i] = fmaxf(umax[i], u);
                     }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    kahan_add(rre[i], kre[i], bre[i]);
+                    kahan_add(rim[i], kim[i], bim[i]);
                 }
                 if (__ballot(dmin <= F.e_band)) {
                     for (int kk = kb; kk < ke; kk++) {
@@ -463,8 +476,8 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                                 // replace the pass's s (u + 1) by the exact test's weight
                                 const float s = step(u);
                                 const float d = (in ? u + 1.0f : 0.0f) - fmaf(s, u, s);
-                                rre[i] = fmaf(d, B.w, rre[i]);
-                                rim[i] = fmaf(d, C.x, rim[i]);
+                                kahan_add(rre[i], kre[i], d * B.w);
+                                kahan_add(rim[i], kim[i], d * C.x);
                             }
                         }
                     }

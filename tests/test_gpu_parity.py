@@ -401,6 +401,8 @@ def test_batch_branches_fit_free_memory(monkeypatch):
     _, im2 = _cond("c0", cells=(10, 10, 10), sg=0.015, density=True)
     x3, im3 = _cond("c1", cells=(12, 9, 7))
 
+    monkeypatch.setenv("D2BI_TUNE", "0")             # every branch that fits runs (no timing)
+
     def run(mem):
         imaging.finalize()                           # drop every workspace
         if mem is None:
@@ -425,6 +427,7 @@ def test_batch_branches_fit_free_memory(monkeypatch):
                 assert np.array_equal(a, b), mem
     finally:
         monkeypatch.delenv("D2BI_BATCH_MEM", raising=False)
+        monkeypatch.delenv("D2BI_TUNE", raising=False)
         imaging.finalize()
 
 
