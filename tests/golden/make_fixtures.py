@@ -8,8 +8,10 @@ Writes, next to this script:
   linkcell_kat.json     the 200 positions and expected outputs of
                         test/testLib_LinkCell3D.f90:66-290
   w_loop_golden.npz     Data/test/test.i111_loop_W.lammps atom positions and
-                        box, and the golden image + header of
-                        Data/test/...g2_0_0.k0.00_0.00_1.00_ng1.00.dat
+                        box, the golden image + header of
+                        Data/test/...g2_0_0.k0.00_0.00_1.00_ng1.00.dat, and
+                        the 16-bit grey levels of the golden
+                        Data/test/...g2_0_0.k0.00_0.00_1.00_ng1.00.png
   aperture_points.json  the table of n = 1..25 points in the unit disk that
                         evenlySpacedPointsInCircle returns (src/evenlySpaced.h),
                         as the decimal literals written there
@@ -20,6 +22,7 @@ import json
 import os
 import re
 import sys
+import zlib
 
 import numpy as np
 
@@ -93,10 +96,59 @@ def w_loop(ref):
     for ln in dl[12:]:
         vals.extend(float(ln[i:i + 12]) for i in range(0, len(ln), 12) if ln[i:i + 12].strip())
     img = np.array(vals).reshape(ny, nx)        # written x fastest: row = iy
+    png = read_png16(dat[:-4] + ".png")
+    assert png.shape == img.shape
     return dict(
         x=xyz, ids=ids, box=np.array(box), modg=float(g_line[-1]), xi=np.array(xi), sg=sg,
         space=np.array(space), scale=scale, rotation_cm=np.array(rot), image=img,
-        header="\n".join(dl[:11]))
+        header="\n".join(dl[:11]), png=png.astype(np.uint16))
+
+
+def read_png16(path):
+    """Grey levels of a 16-bit greyscale, non-interlaced PNG as an (h, w)
+    array of int64 (row 0 first): the IDAT stream inflated and each row's
+    filter (none, sub, up, average, Paeth; 2 bytes per pixel) undone, as
+    the PNG specification defines them."""
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat = 8, b""
+    while pos < len(data):
+        n = int.from_bytes(data[pos:pos + 4], "big")
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        if typ == b"IHDR":
+            w, h = int.from_bytes(body[:4], "big"), int.from_bytes(body[4:8], "big")
+            assert body[8] == 16 and body[9] == 0 and body[12] == 0, "16-bit grey, not interlaced"
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = zlib.decompress(idat)
+    stride, bpp = 2 * w, 2
+    prev = [0] * stride
+    rows = []
+    for y in range(h):
+        f = raw[y * (stride + 1)]
+        line = raw[y * (stride + 1) + 1:(y + 1) * (stride + 1)]
+        cur = [0] * stride
+        for i in range(stride):
+            a = cur[i - bpp] if i >= bpp else 0
+            b = prev[i]
+            c = prev[i - bpp] if i >= bpp else 0
+            if f == 0:
+                p = 0
+            elif f == 1:
+                p = a
+            elif f == 2:
+                p = b
+            elif f == 3:
+                p = (a + b) // 2
+            else:
+                pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+            cur[i] = (line[i] + p) & 255
+        rows.append(cur)
+        prev = cur
+    v = np.array(rows, np.int64).reshape(h, w, 2)
+    return v[:, :, 0] * 256 + v[:, :, 1]
 
 
 def aperture_points(ref):

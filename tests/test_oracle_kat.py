@@ -169,6 +169,33 @@ def test_blur_properties():
     assert np.allclose(orc.blur(r, 1.7).T, orc.blur(r.T, 1.7), atol=1e-15)
 
 
+def png_levels(img, blur_px):
+    """outputImgAsPng with png_min = png_max (d2bi.f90:1509-1538): blur by
+    blur_px pixels, normalise (Lib_Png.f90:138-151, ONE = 1 - 1/65536), round
+    to FP32, clamp and scale to 16 bits (Lib_Greyscale.c:234-243)."""
+    b = orc.blur(img, blur_px) if blur_px > 0 else img
+    aa = (1.0 - 1.0 / 65536) / (b.max() - b.min())
+    f = (aa * b - aa * b.min()).astype(np.float32)
+    f = np.clip(f, np.float32(0.0), np.float32(0.999984741))
+    return (f * np.float32(65536)).astype(np.int64)
+
+
+def test_oracle_blur_reproduces_golden_png(golden_dir):
+    """The reference's own Data/test/...ng1.00.png (94x94, 16-bit grey) is
+    its .dat image blurred by png_blur / scale = 2.5 A / 0.79502 A px^-1 and
+    normalised.  The oracle's blur of the .dat image (written to 7
+    decimals) gives the same grey levels to within one level; without the
+    blur, or with the width in A rather than pixels, they are far off."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    ref = w["png"].astype(np.int64)
+    t = 2.5 / float(w["scale"])
+    d = np.abs(png_levels(w["image"], t) - ref)
+    print("golden png: max %d levels, %.4f of pixels exact" % (d.max(), np.mean(d == 0)))
+    assert d.max() <= 1 and np.mean(d == 0) >= 0.98
+    assert np.abs(png_levels(w["image"], 0.0) - ref).max() > 1000
+    assert np.abs(png_levels(w["image"], 2.5) - ref).max() > 100
+
+
 def test_oracle_reproduces_golden_image(golden_dir):
     """The reference's own output Data/test/...ng1.00.dat (94x94, W loop).
     Reproduced through -U <header rotation> -R I -d <header image space>;
