@@ -10,6 +10,10 @@ module d2bi_gpu_mod
     integer(c_int), parameter, public :: D2BI_MODE_FAST = 1
     integer, parameter, public        :: D2BI_MAX_UVW = 125
     integer, parameter, public        :: D2BI_MAX_PREC = 256
+    integer, parameter, public        :: D2BI_MAX_APERTURE = 1024
+    integer(c_int64_t), parameter, public :: D2BI_SCRATCH_BYTES = 67108864_c_int64_t
+    integer(c_int), parameter, public :: D2BI_OK = 0, D2BI_ERR_ARG = -1, D2BI_ERR_HIP = -2, &
+                                         D2BI_ERR_CAPACITY = -3, D2BI_ERR_NOINIT = -4, D2BI_ERR_NODEVICE = -5
 
     !   type DynamicalTwoBeamImaging fields read by computeD2BI_img
     !   (Lib_DynamicalTwoBeamImaging.f90:64-90)
@@ -48,10 +52,23 @@ module d2bi_gpu_mod
         integer(c_int32_t) :: n_branches, pad0
     end type d2bi_stats
 
-    public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_plan_image
+    !   d2bi_synth_bcc parameters (no counterpart in the reference)
+    type, bind(c), public :: d2bi_synth_params
+        real(c_double)     :: a0
+        integer(c_int32_t) :: nx, ny, nz, n_centres
+        real(c_double)     :: A_min, A_max, u_cap
+        integer(c_int32_t) :: n_voids, pad0
+        real(c_double)     :: r_min, r_max, noise, offset(3)
+        integer(c_int64_t) :: seed
+    end type d2bi_synth_params
+
+    public :: d2bi_gpu_init, d2bi_gpu_finalize, d2bi_last_error, d2bi_version, d2bi_plan_image
     public :: d2bi_setup_tilt_sg, d2bi_setup_tilt_ur, d2bi_setup_tilt_ng, d2bi_rotate_to_standard_frame
-    public :: d2bi_gpu_atoms_upload, d2bi_gpu_atoms_free, d2bi_gpu_image_dev, d2bi_gpu_blur
+    public :: d2bi_aperture_points, d2bi_aperture_setup, d2bi_wavelength, d2bi_velocity, d2bi_deviation_parameter
+    public :: d2bi_gpu_atoms_upload, d2bi_gpu_atoms_free, d2bi_gpu_atoms_count
+    public :: d2bi_gpu_image, d2bi_gpu_image_dev, d2bi_gpu_image_dev_out, d2bi_gpu_image_noalloc
     public :: d2bi_gpu_image_aperture, d2bi_gpu_image_batch, d2bi_gpu_image_batch_dev_out
+    public :: d2bi_gpu_field, d2bi_gpu_linkcells, d2bi_gpu_lc_query, d2bi_gpu_blur, d2bi_synth_bcc
     public :: d2bi_write_png16, d2bi_error_message
 
     interface
@@ -118,6 +135,118 @@ module d2bi_gpu_mod
         integer(c_int) function d2bi_gpu_atoms_free(atoms) bind(c, name="d2bi_gpu_atoms_free")
             import :: c_int, c_ptr
             type(c_ptr), value :: atoms
+        end function
+        integer(c_int64_t) function d2bi_gpu_atoms_count(atoms) bind(c, name="d2bi_gpu_atoms_count")
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: atoms
+        end function
+        ! computeD2BI_img with host positions x(ld, nAtoms), first three rows read
+        integer(c_int) function d2bi_gpu_image(rec, x, ld, nAtoms, precAngle, nPrec, mode, img, stats) &
+                bind(c, name="d2bi_gpu_image")
+            import :: c_int, c_int32_t, c_int64_t, c_double, d2bi_record, d2bi_stats
+            type(d2bi_record), intent(in)   :: rec
+            real(c_double), intent(in)      :: x(*)
+            integer(c_int64_t), value       :: ld, nAtoms
+            real(c_double), value           :: precAngle
+            integer(c_int32_t), value       :: nPrec, mode
+            real(c_double), intent(out)     :: img(*)
+            type(d2bi_stats), intent(out)   :: stats
+        end function
+        ! the image stays in HBM: d_img is a device address of Mx*My doubles
+        integer(c_int) function d2bi_gpu_image_dev_out(rec, atoms, precAngle, nPrec, mode, d_img, stats) &
+                bind(c, name="d2bi_gpu_image_dev_out")
+            import :: c_int, c_int32_t, c_double, c_ptr, d2bi_record, d2bi_stats
+            type(d2bi_record), intent(in)   :: rec
+            type(c_ptr), value              :: atoms
+            real(c_double), value           :: precAngle
+            integer(c_int32_t), value       :: nPrec, mode
+            type(c_ptr), value              :: d_img
+            type(d2bi_stats), intent(out)   :: stats
+        end function
+        ! as d2bi_gpu_image_dev_out with no device allocation (workspace: a
+        ! static block of D2BI_SCRATCH_BYTES per device)
+        integer(c_int) function d2bi_gpu_image_noalloc(rec, atoms, precAngle, nPrec, mode, d_img, stats) &
+                bind(c, name="d2bi_gpu_image_noalloc")
+            import :: c_int, c_int32_t, c_double, c_ptr, d2bi_record, d2bi_stats
+            type(d2bi_record), intent(in)   :: rec
+            type(c_ptr), value              :: atoms
+            real(c_double), value           :: precAngle
+            integer(c_int32_t), value       :: nPrec, mode
+            type(c_ptr), value              :: d_img
+            type(d2bi_stats), intent(out)   :: stats
+        end function
+        ! voxel field; each output may be c_null_ptr (else c_loc of an array)
+        integer(c_int) function d2bi_gpu_field(rec, x, ld, nAtoms, mode, phase_re_im, rho, count, r2min) &
+                bind(c, name="d2bi_gpu_field")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, d2bi_record
+            type(d2bi_record), intent(in)   :: rec
+            real(c_double), intent(in)      :: x(*)
+            integer(c_int64_t), value       :: ld, nAtoms
+            integer(c_int32_t), value       :: mode
+            type(c_ptr), value              :: phase_re_im, rho, count, r2min
+        end function
+        ! findGdotR's link cells; each output may be c_null_ptr
+        integer(c_int) function d2bi_gpu_linkcells(rec, x, ld, nAtoms, cell_start, ids, frac, gdotr, cap, &
+                nPoints) bind(c, name="d2bi_gpu_linkcells")
+            import :: c_int, c_int64_t, c_double, c_ptr, d2bi_record
+            type(d2bi_record), intent(in)   :: rec
+            real(c_double), intent(in)      :: x(*)
+            integer(c_int64_t), value       :: ld, nAtoms
+            type(c_ptr), value              :: cell_start, ids, frac, gdotr
+            integer(c_int64_t), value       :: cap
+            integer(c_int64_t), intent(out) :: nPoints
+        end function
+        ! LinkCell3D_ctor0 + add1 + neighbourList1b for nQ query points
+        integer(c_int) function d2bi_gpu_lc_query(a, Nx, Ny, Nz, p, nP, q, nQ, rc, maxNeigh, n, id, r2) &
+                bind(c, name="d2bi_gpu_lc_query")
+            import :: c_int, c_int32_t, c_int64_t, c_double
+            real(c_double), intent(in)      :: a(9)
+            integer(c_int32_t), value       :: Nx, Ny, Nz
+            real(c_double), intent(in)      :: p(*)
+            integer(c_int64_t), value       :: nP
+            real(c_double), intent(in)      :: q(*)
+            integer(c_int64_t), value       :: nQ
+            real(c_double), value           :: rc
+            integer(c_int32_t), value       :: maxNeigh
+            integer(c_int32_t), intent(out) :: n(*), id(*)
+            real(c_double), intent(out)     :: r2(*)
+        end function
+        integer(c_int) function d2bi_aperture_points(n, xy) bind(c, name="d2bi_aperture_points")
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value   :: n
+            real(c_double), intent(out) :: xy(*)
+        end function
+        integer(c_int) function d2bi_aperture_setup(rec, precAngle, nPrec, apertureAngle, nAperture, gshift, aa) &
+                bind(c, name="d2bi_aperture_setup")
+            import :: c_int, c_int32_t, c_double, d2bi_record
+            type(d2bi_record), intent(in) :: rec
+            real(c_double), value         :: precAngle, apertureAngle
+            integer(c_int32_t), value     :: nPrec, nAperture
+            real(c_double), intent(out)   :: gshift(*), aa(*)
+        end function
+        real(c_double) function d2bi_wavelength(V) bind(c, name="d2bi_wavelength")
+            import :: c_double
+            real(c_double), value :: V
+        end function
+        real(c_double) function d2bi_velocity(V) bind(c, name="d2bi_velocity")
+            import :: c_double
+            real(c_double), value :: V
+        end function
+        real(c_double) function d2bi_deviation_parameter(E_keV, g_lab) bind(c, name="d2bi_deviation_parameter")
+            import :: c_double
+            real(c_double), value      :: E_keV
+            real(c_double), intent(in) :: g_lab(3)
+        end function
+        integer(c_int) function d2bi_version() bind(c, name="d2bi_version")
+            import :: c_int
+        end function
+        integer(c_int) function d2bi_synth_bcc(p, x, cap, nAtoms, box) bind(c, name="d2bi_synth_bcc")
+            import :: c_int, c_int64_t, c_double, d2bi_synth_params
+            type(d2bi_synth_params), intent(in) :: p
+            real(c_double), intent(out)         :: x(*)
+            integer(c_int64_t), value           :: cap
+            integer(c_int64_t), intent(out)     :: nAtoms
+            real(c_double), intent(out)         :: box(3)
         end function
         integer(c_int) function d2bi_gpu_image_dev(rec, atoms, precAngle, nPrec, mode, img, stats) &
                 bind(c, name="d2bi_gpu_image_dev")

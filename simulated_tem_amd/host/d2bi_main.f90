@@ -475,22 +475,6 @@ contains
         a_vox(:, 3) = a_super(:, 3) / n3(3)
     end subroutine voxel_super
 
-    !   wrapPBC0 (Lib_SimpleSupercells.f90:992-1021) for a cuboidal cell
-    function wrap_pbc(a, n3, y) result(yp)
-        real(real64), intent(in) :: a(3, 3), y(3)
-        integer, intent(in)      :: n3(3)
-        real(real64) :: yp(3), xx(3)
-        integer :: i, k
-        do i = 1, 3
-            xx(i) = y(i) / a(i, i)
-            k = floor(xx(i))
-            xx(i) = xx(i) - k
-            k = mod(k + ishft(n3(i), 8), n3(i))
-            xx(i) = k + xx(i)
-            yp(i) = a(i, i) * xx(i)
-        end do
-    end function wrap_pbc
-
     !   -xpad/-ypad/-zpad (d2bi.f90:1036-1082)
     subroutine pad_axis(ax)
         integer, intent(in) :: ax

@@ -482,6 +482,43 @@ def test_golden_w_loop_image(golden_dir):
         assert abs(got - want) <= 0.0039
 
 
+SHEAR = np.array([[1.0, 0.04, 0.02], [0.0, 1.0, 0.03], [0.0, 0.0, 1.0]])
+
+
+def test_triclinic_supercell_w_loop(golden_dir):
+    """A sheared (triclinic) periodic supercell: the W loop's atoms and box
+    under x -> S x, S upper unitriangular (LAMMPS tilt factors xy, xz, yz =
+    0.04 ly, 0.02 lz, 0.03 lz).  findGdotR's 27 replicas are translations
+    along the sheared vectors (getSuperVector, :904-918); FP64 and FAST
+    against the oracle."""
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    a0 = lattice_a0(3.1652)
+    box = w["box"][:, 1] - w["box"][:, 0]
+    sa, sn = input_supercell(SHEAR @ np.diag(box), 4, a0)
+    assert sa[0, 1] != 0 and sa[0, 2] != 0 and sa[1, 2] != 0
+    im = DynamicalTwoBeamImaging(sa, sn, a0, (0, 0, 1), (2, 0, 0), 200.0, 4, 4)
+    im.setConventionalUnitCell(a0 * np.eye(3))
+    im.setVoxelSupercellSize(w["space"])
+    im.setSupercellTilt(w["rotation_cm"].reshape(3, 3).T, np.eye(3))
+    im.setExtinctionDistances(*w["xi"])
+    im.setUseDensityField(False)
+    x = w["x"] @ SHEAR.T
+    g64 = im.computeD2BI_img(x, mode="fp64")
+    gf = im.computeD2BI_img(x, mode="fast")
+    ref = orc.Condition(im.rec, x).image(nthreads=NTH)
+    e64, ef = np.max(np.abs(g64 - ref)), np.max(np.abs(gf - ref))
+    print("[triclinic] fp64 %.3e fast %.3e  I in [%.4f, %.4f]" % (e64, ef, ref.min(), ref.max()), flush=True)
+    assert e64 <= FP64_TOL and ef <= FAST_TOL
+    # the shear changes the image (the replicas move)
+    plain = DynamicalTwoBeamImaging(*input_supercell(np.diag(box), 4, a0), a0, (0, 0, 1), (2, 0, 0), 200.0, 4, 4)
+    plain.setConventionalUnitCell(a0 * np.eye(3))
+    plain.setVoxelSupercellSize(w["space"])
+    plain.setSupercellTilt(w["rotation_cm"].reshape(3, 3).T, np.eye(3))
+    plain.setExtinctionDistances(*w["xi"])
+    plain.setUseDensityField(False)
+    assert np.max(np.abs(plain.computeD2BI_img(w["x"], mode="fp64") - g64)) > 1e-3
+
+
 def test_c2_full_size_density_parity():
     """c2 at full size: 640x640x1600 voxels, 20.5e6 atoms, g = 200, density
     field on, crystal coordinates up to ~1.4e3 A (g.r up to ~5e3 rad).  All

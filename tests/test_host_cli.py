@@ -18,11 +18,13 @@ def run(args, cwd):
     return subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
 
 
-def write_lammps(path, x, box):
+def write_lammps(path, x, box, tilt=None):
     with open(path, "w") as f:
         f.write("LAMMPS data file written by the d2bi host tests\n\n%d atoms\n1 atom types\n\n" % len(x))
         for i, ax in enumerate("xyz"):
             f.write("%.10f %.10f %slo %shi\n" % (box[i][0], box[i][1], ax, ax))
+        if tilt is not None:
+            f.write("%.10f %.10f %.10f xy xz yz\n" % tuple(tilt))
         f.write("\nMasses\n\n1 183.84\n\nAtoms # atomic\n\n")
         for i, p in enumerate(x):
             f.write("%d 1 %.16g %.16g %.16g 0 0 0\n" % (i + 1, p[0], p[1], p[2]))
@@ -178,3 +180,56 @@ def test_tomography_series_batched_equals_sequential(tmp_path, golden_dir):
             assert a.splitlines()[1:] == b.splitlines()[1:]       # first line names the file
         assert outs["batch"][0].splitlines()[-5:] != outs["batch"][2].splitlines()[-5:]
         print("[tomography] %s: batch == sequential for 3 tilts" % mode, flush=True)
+
+
+def _read_dat(path):
+    lines = open(path).read().splitlines()
+    h = next(i for i, ln in enumerate(lines) if not ln.startswith("#"))
+    nx, ny = (int(v) for v in lines[h].split())
+    vals = [float(ln[i:i + 12]) for ln in lines[h + 1:] for i in range(0, len(ln), 12) if ln[i:i + 12].strip()]
+    return np.array(vals).reshape(ny, nx)
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_triclinic_lammps_run_through_host(tmp_path, golden_dir):
+    """A triclinic LAMMPS input (the W loop sheared by S, upper
+    unitriangular; tilt factors on the "xy xz yz" line): the host reads
+    the tilted supercell, wraps the atoms into it and writes the image of
+    the library's Python binding on the same supercell, to the .dat's 7
+    decimals."""
+    from simulated_tem_amd.imaging import DynamicalTwoBeamImaging, input_supercell, lattice_a0
+    S = np.array([[1.0, 0.04, 0.02], [0.0, 1.0, 0.03], [0.0, 0.0, 1.0]])
+    w = np.load(os.path.join(golden_dir, "w_loop_golden.npz"))
+    box = w["box"][:, 1] - w["box"][:, 0]
+    lo = np.zeros(3)
+    A = S @ np.diag(box)
+    x = (w["x"] - w["box"][:, 0]) @ S.T
+    write_lammps(tmp_path / "t.lammps", x, list(zip(lo, box)), tilt=(A[0, 1], A[0, 2], A[1, 2]))
+    write_xi(tmp_path / "xi.dat", w["xi"][0], w["xi"][1], (2, 0, 0))
+    U = ",".join("%.6f" % v for v in w["rotation_cm"])
+    d = ",".join("%.3f" % v for v in w["space"])
+    r = run(["-f", "t.lammps", "-a0", "3.1652", "-lattice", "bcc", "-k", "0,0,1", "-g", "2,0,0", "-xifile", "xi.dat",
+             "-U", U, "-R", "1,0,0,0,1,0,0,0,1", "-d", d, "-o", "tri", "-mode", "fp64", "-nopng"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dats = [f for f in os.listdir(tmp_path) if f.startswith("tri") and f.endswith(".dat")]
+    assert len(dats) == 1
+    got = _read_dat(tmp_path / dats[0])
+    a0 = lattice_a0(3.1652)
+    # the supercell and positions as the file holds them
+    r10 = lambda v: float("%.10f" % v)
+    A_file = np.array([[r10(box[0]), 0, 0], [r10(A[0, 1]), r10(box[1]), 0], [r10(A[0, 2]), r10(A[1, 2]), r10(box[2])]]).T
+    x = np.array([[float("%.16g" % v) for v in p] for p in x])
+    sa, sn = input_supercell(A_file, 4, a0)
+    im = DynamicalTwoBeamImaging(sa, sn, a0, (0, 0, 1), (2, 0, 0), 200.0, 4, 4)
+    im.setConventionalUnitCell(a0 * np.eye(3))
+    im.setVoxelSupercellSize([float(v) for v in d.split(",")])
+    im.setSupercellTilt(np.array([float(v) for v in U.split(",")]).reshape(3, 3).T, np.eye(3))
+    im.setExtinctionDistances(w["xi"][0], w["xi"][1])
+    im.setUseDensityField(False)
+    want = im.computeD2BI_img(x, mode="fp64")
+    e = np.max(np.abs(got - want))
+    print("[triclinic host] max |dat - image| %.3e" % e, flush=True)
+    assert got.shape == want.shape
+    assert e <= 6e-8
+
