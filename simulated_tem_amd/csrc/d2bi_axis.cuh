@@ -26,10 +26,16 @@ struct QL {
     int lc;
 };
 
+// Voxel positions relative to the first link cell of their tile: table k
+// holds axis QL_AXIS[k] for tiles of QL_TILE[k] voxels along it.
+static constexpr int NQL = 5;
+static constexpr int QL_AXIS[NQL] = {0, 1, 2, 2, 0};
+static constexpr int QL_TILE[NQL] = {4, 4, 4, 8, 8};
+
 struct AxisTables {
     std::vector<int> cell[3];
     std::vector<double> zi[3];
-    std::vector<QL> ql[4];         // x, y, z for 4-voxel tiles; z for 8-voxel tiles
+    std::vector<QL> ql[NQL];       // x, y, z for 4-voxel tiles; z, x for 8-voxel tiles
     std::vector<int> start[3];     // first voxel of each link cell (size E+1)
     int lo[3], hi[3];              // active link-cell range
 };
@@ -60,18 +66,14 @@ static int axis_tables(const d2bi_record* r, const d2bi_plan* p, AxisTables* t)
         for (int i = 1; i < M[a]; i++)
             if (t->cell[a][i] < t->cell[a][i - 1])
                 return seterr(D2BI_ERR_ARG, "voxel column wraps the link-cell box (axis %d)", a);
-        t->ql[a].resize(M[a]);
         const double side = p->a_col[4 * a];
-        for (int i = 0; i < M[a]; i++) {
-            const int lc = t->cell[a][i] - t->cell[a][(i / TILE) * TILE];
-            t->ql[a][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
-        }
-        if (a == 2) {
-            // z again, relative to the cell of 8-voxel tile starts (4 x 4 x 8 tiles)
-            t->ql[3].resize(M[a]);
+        for (int k = 0; k < NQL; k++) {
+            if (QL_AXIS[k] != a) continue;
+            const int ts = QL_TILE[k];
+            t->ql[k].resize(M[a]);
             for (int i = 0; i < M[a]; i++) {
-                const int lc = t->cell[a][i] - t->cell[a][(i / 8) * 8];
-                t->ql[3][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
+                const int lc = t->cell[a][i] - t->cell[a][(i / ts) * ts];
+                t->ql[k][i] = QL{(float)(side * (t->zi[a][i] + lc)), lc};
             }
         }
         for (int i = M[a] - 1; i >= 0; i--) t->start[a][t->cell[a][i]] = i;
@@ -89,7 +91,7 @@ struct AxisDev {
     const int* cell[3];
     const double* zi[3];
     const int* start[3];
-    const QL* ql[4];
+    const QL* ql[NQL];
 };
 
 // Copy the axis tables into workspace w.  t must stay alive until the
@@ -98,7 +100,8 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s, Workspace
 {
     size_t nc = t.cell[0].size() + t.cell[1].size() + t.cell[2].size();
     size_t ns = t.start[0].size() + t.start[1].size() + t.start[2].size();
-    size_t nq = nc + t.ql[3].size();
+    size_t nq = 0;
+    for (int k = 0; k < NQL; k++) nq += t.ql[k].size();
     CHK(w.ax_cell.ensure(sizeof(int) * nc));
     CHK(w.ax_zi.ensure(sizeof(double) * nc));
     CHK(w.ax_start.ensure(sizeof(int) * ns));
@@ -106,14 +109,17 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s, Workspace
     if (w.ax_cell.arena) {
         // a workspace that must not allocate: copy from the tables
         // themselves, not through the (growable) pinned staging buffer
-        size_t oc = 0, os = 0;
+        size_t oc = 0, os = 0, oq = 0;
+        for (int k = 0; k < NQL; k++) {
+            d->ql[k] = w.ax_ql.as<QL>() + oq;
+            HIPCHK(hipMemcpyAsync((void*)d->ql[k], t.ql[k].data(), sizeof(QL) * t.ql[k].size(),
+                                  hipMemcpyHostToDevice, s));
+            oq += t.ql[k].size();
+        }
         for (int a = 0; a < 3; a++) {
-            d->ql[a] = w.ax_ql.as<QL>() + oc;
             d->cell[a] = w.ax_cell.as<int>() + oc;
             d->zi[a] = w.ax_zi.as<double>() + oc;
             d->start[a] = w.ax_start.as<int>() + os;
-            HIPCHK(hipMemcpyAsync((void*)d->ql[a], t.ql[a].data(), sizeof(QL) * t.ql[a].size(),
-                                  hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync((void*)d->cell[a], t.cell[a].data(), sizeof(int) * t.cell[a].size(),
                                   hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync((void*)d->zi[a], t.zi[a].data(), sizeof(double) * t.zi[a].size(),
@@ -123,9 +129,6 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s, Workspace
             oc += t.cell[a].size();
             os += t.start[a].size();
         }
-        d->ql[3] = w.ax_ql.as<QL>() + nc;
-        HIPCHK(hipMemcpyAsync((void*)d->ql[3], t.ql[3].data(), sizeof(QL) * t.ql[3].size(), hipMemcpyHostToDevice,
-                              s));
         return D2BI_OK;
     }
     size_t bytes = sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns + sizeof(QL) * nq;
@@ -136,17 +139,16 @@ static int upload_axis(const AxisTables& t, AxisDev* d, hipStream_t s, Workspace
     double* hz = (double*)(h + sizeof(int) * nc);
     int* hs = (int*)(h + sizeof(int) * nc + sizeof(double) * nc);
     QL* hq = (QL*)(h + sizeof(int) * nc + sizeof(double) * nc + sizeof(int) * ns);
-    size_t oc = 0, os = 0;
+    size_t oc = 0, os = 0, oq = 0;
+    for (int k = 0; k < NQL; k++) {
+        std::memcpy(hq + oq, t.ql[k].data(), sizeof(QL) * t.ql[k].size());
+        d->ql[k] = w.ax_ql.as<QL>() + oq;
+        oq += t.ql[k].size();
+    }
     for (int a = 0; a < 3; a++) {
         std::memcpy(hc + oc, t.cell[a].data(), sizeof(int) * t.cell[a].size());
         std::memcpy(hz + oc, t.zi[a].data(), sizeof(double) * t.zi[a].size());
         std::memcpy(hs + os, t.start[a].data(), sizeof(int) * t.start[a].size());
-        std::memcpy(hq + oc, t.ql[a].data(), sizeof(QL) * t.ql[a].size());
-        d->ql[a] = w.ax_ql.as<QL>() + oc;
-        if (a == 2) {
-            std::memcpy(hq + nc, t.ql[3].data(), sizeof(QL) * t.ql[3].size());
-            d->ql[3] = w.ax_ql.as<QL>() + nc;
-        }
         d->cell[a] = w.ax_cell.as<int>() + oc;
         d->zi[a] = w.ax_zi.as<double>() + oc;
         d->start[a] = w.ax_start.as<int>() + os;

@@ -256,7 +256,7 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             c.o_mask = o_mask; o_mask += std::max<int64_t>(c.n, 1);
             c.o_blk = o_blk; o_blk += c.nbc + 1;
             c.o_ax = o_ax; o_ax += c.plan.Mx + c.plan.My + c.plan.Mz;
-            c.o_axq = o_axq; o_axq += c.plan.Mx + c.plan.My + 2 * (int64_t)c.plan.Mz;
+            c.o_axq = o_axq; o_axq += 2 * (int64_t)c.plan.Mx + c.plan.My + 2 * (int64_t)c.plan.Mz;
             c.o_axs = o_axs; o_axs += c.plan.Ex + c.plan.Ey + c.plan.Ez + 3;
             c.o_img = o_img; o_img += c.plane;
             c.o_aa = o_aa; o_aa += c.plan.nPrec;
@@ -282,10 +282,10 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             for (int i = 0; i < nRec; i++) {
                 Cond& c = C[i];
                 int64_t oc = c.o_ax, os = c.o_axs, oq = c.o_axq;
-                for (int a = 0; a < 4; a++) {
-                    std::copy(c.tab.ql[a].begin(), c.tab.ql[a].end(), hq.begin() + oq);
-                    c.ax.ql[a] = B.axq.as<QL>() + oq;
-                    oq += (int64_t)c.tab.ql[a].size();
+                for (int k = 0; k < NQL; k++) {
+                    std::copy(c.tab.ql[k].begin(), c.tab.ql[k].end(), hq.begin() + oq);
+                    c.ax.ql[k] = B.axq.as<QL>() + oq;
+                    oq += (int64_t)c.tab.ql[k].size();
                 }
                 for (int a = 0; a < 3; a++) {
                     std::copy(c.tab.cell[a].begin(), c.tab.cell[a].end(), hc.begin() + oc);

@@ -39,14 +39,16 @@ static void fill_vox_params(const d2bi_record* r, const d2bi_plan* p, const Axis
 // Tile depth of a FAST kernel variant: 0 (a lane per voxel) and 1 (rows of
 // 4 x 4 x 4 tiles) use 4 voxels, 2 (rows of 4 x 4 x 8 tiles) 8.
 static int variant_tz(int variant) { return variant == 2 ? 8 : 4; }
+// Voxels along x of a variant's tiles: 8 for variant 3 (rows of 8 x 4 x 4 tiles).
+static int variant_tx(int variant) { return variant == 3 ? 8 : 4; }
 
-// Link cells in the largest unwrapped cell range of a 4 x 4 x tz tile.
-static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz)
+// Link cells in the largest unwrapped cell range of a tx x 4 x tz tile.
+static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz, int tx = TILE)
 {
     int worst = 1;
     for (int a = 0; a < 3; a++) {
         const int M = a == 0 ? p->Mx : a == 1 ? p->My : p->Mz;
-        const int ts = a == 2 ? tz : TILE;
+        const int ts = a == 2 ? tz : a == 0 ? tx : TILE;
         int mx = 0;
         for (int t0 = 0; t0 < M; t0 += ts)
             mx = std::max(mx, tab.cell[a][std::min(M, t0 + ts) - 1] - tab.cell[a][t0] + 3);
@@ -55,13 +57,14 @@ static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz)
     return worst;
 }
 
-// Launch parameters of the FAST voxel kernel for a plan, tiles of 4 x 4 x tz.
-static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo, int tz)
+// Launch parameters of the FAST voxel kernel for a plan, tiles of tx x 4 x tz.
+static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, FastParams* Fo, int tz,
+                       int tx = TILE)
 {
     FastParams& F = *Fo;
     F.v = V;
     F.tz = tz;
-    F.ntx = (p->Mx + TILE - 1) / TILE;
+    F.ntx = (p->Mx + tx - 1) / tx;
     F.nty = (p->My + TILE - 1) / TILE;
     F.ntz = (p->Mz + tz - 1) / tz;
     F.ntiles = F.ntx * F.nty * F.ntz;
@@ -86,28 +89,37 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
                       p->a_col[4], rc);
     F.thin_z = p->a_col[8] < thin ? 1 : 0;
     // the unwrapped cell range of a tile must fit the per-tile cell table
-    const int worst = tile_cells(p, tab, tz);
+    const int worst = tile_cells(p, tab, tz, tx);
     if (worst > TILE_MAXC) return seterr(D2BI_ERR_CAPACITY, "voxel tile spans %d link cells (max %d)", worst, TILE_MAXC);
     return D2BI_OK;
 }
 
 // Launch parameters of a variant.  A 4 x 4 x 8 tile that spans more link
 // cells than the per-tile table holds falls back to the 4 x 4 x 4 row kernel
-// (*variant becomes 1).
+// (*variant becomes 1), and so does an 8 x 4 x 4 tile (variant 3) that
+// spans too many.
+static int effective_variant(const d2bi_plan* p, const AxisTables& tab, int variant)
+{
+    if (variant == 3 && tile_cells(p, tab, 4, 8) > TILE_MAXC) variant = 1;
+    if (variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) variant = 1;
+    return variant;
+}
+
 static int variant_params(const d2bi_plan* p, const AxisTables& tab, const VoxParams& V, int* variant,
                           FastParams* F)
 {
-    if (*variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) *variant = 1;
-    return fast_params(p, tab, V, F, variant_tz(*variant));
+    *variant = effective_variant(p, tab, *variant);
+    return fast_params(p, tab, V, F, variant_tz(*variant), variant_tx(*variant));
 }
 
 // Kernel instance for (diagnostics, density, variant); link cells thinner
-// than 3 sigma along z take a row kernel (variant 1 or 2) with the z-offset
-// test.
+// than 3 sigma along z take a row kernel (variant 1, 2 or 3) with the
+// z-offset test.
 template <bool DENS, int ROW, class Go>
-static void pick_row(int tz, Go go)
+static void pick_row(int tz, int tx, Go go)
 {
-    if (tz == 8) go(k_voxel_tile<DENS, false, ROW, 8>);
+    if (tx == 8) go(k_voxel_tile<DENS, false, ROW, 4, 8>);
+    else if (tz == 8) go(k_voxel_tile<DENS, false, ROW, 8>);
     else go(k_voxel_tile<DENS, false, ROW, 4>);
 }
 
@@ -115,16 +127,16 @@ template <class Go>
 static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
 {
     if (diag) { go(k_voxel_tile<true, true, 0, 4>); return; }
-    const int tz = variant_tz(variant);
+    const int tz = variant_tz(variant), tx = variant_tx(variant);
     if (!thin_z && variant == 0) {
         if (dens) go(k_voxel_tile<true, false, 0, 4>);
         else go(k_voxel_tile<false, false, 0, 4>);
     } else if (thin_z) {
-        if (dens) pick_row<true, 2>(tz, go);
-        else pick_row<false, 2>(tz, go);
+        if (dens) pick_row<true, 2>(tz, tx, go);
+        else pick_row<false, 2>(tz, tx, go);
     } else {
-        if (dens) pick_row<true, 1>(tz, go);
-        else pick_row<false, 1>(tz, go);
+        if (dens) pick_row<true, 1>(tz, tx, go);
+        else pick_row<false, 1>(tz, tx, go);
     }
 }
 
@@ -160,7 +172,7 @@ static bool tune_variants()
 static int forced_variant()
 {
     const char* env = std::getenv("D2BI_FAST_VARIANT");
-    if (env && env[0] >= '0' && env[0] <= '2' && env[1] == 0) return env[0] - '0';
+    if (env && env[0] >= '0' && env[0] <= '3' && env[1] == 0) return env[0] - '0';
     return -1;
 }
 
@@ -189,14 +201,14 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, T
     // each variant twice, the second (warm-cache) timing counts; thin z
     // cells have only the row kernels
     const int v0 = thin_z ? 1 : 0;
-    float t[3] = {0.0f, 0.0f, 0.0f};
+    float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int rep = 0; rep < 2; rep++)
-        for (int v = v0; v < 3; v++) {
+        for (int v = v0; v < 4; v++) {
             t[v] = time_variant(v);
             if (t[v] < 0) return seterr(D2BI_ERR_HIP, "variant timing failed");
         }
     int best = v0;
-    for (int v = v0 + 1; v < 3; v++)
+    for (int v = v0 + 1; v < 4; v++)
         if (t[v] < t[best]) best = v;
     cache[key] = best;
     return best;
@@ -206,7 +218,7 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, T
 // that runs (after the fallback of variant_params).
 static TuneKey tile_order_key(const d2bi_plan* p, const AxisTables& tab, bool dens, int variant)
 {
-    const int v = (variant == 2 && tile_cells(p, tab, 8) > TILE_MAXC) ? 1 : variant;
+    const int v = effective_variant(p, tab, variant);
     return TuneKey{TUNE_TILE_ORDER, p->Mx, p->My, p->Mz, p->Ex, p->Ey, p->Ez, dens ? 1 : 0, v};
 }
 
@@ -263,7 +275,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     };
     const int variant = fast_variant(p, dens, want_diag, F.thin_z != 0, [&](int v) -> float {
         // a variant whose tiles span too many link cells is not timed
-        if (tile_cells(p, tab, variant_tz(v)) > TILE_MAXC) return 3.4e38f;
+        if (tile_cells(p, tab, variant_tz(v), variant_tx(v)) > TILE_MAXC) return 3.4e38f;
         HIPCHK(hipEventRecord(g.ev[7], s));
         CHK(run_variant(v));
         HIPCHK(hipEventRecord(g.ev[8], s));

@@ -81,18 +81,18 @@ struct FastParams {
 //
 // ROW = true: the wave is the tile's voxel rows along x (one per (y, z): 16
 // for TZ = 4, 32 for TZ = 8) times NQ candidate parts (4 or 2); lane (row, s)
-// walks candidates s, s+NQ, s+2NQ, ... for the four voxels of its row.  The
-// Gaussian factorises along x:
+// walks candidates s, s+NQ, s+2NQ, ... for the TX voxels of its row (TX = 4,
+// or 8 for 8 x 4 x 4 tiles).  The Gaussian factorises along x:
 //     2^(e_i - e_rc) = X_i * 2^t,   X_i = 2^(k2 (p_x - q_x,i)^2),
 //     t = k2 ((p_y - q_y)^2 + (p_z - q_z)^2) - e_rc   (e_rc = k2 (3 sigma)^2),
-// and the four x positions q_x,i are the tile's, common to every row, so
-// X_0..X_3 are evaluated once per candidate when it is staged and a lane
-// spends one v_exp_f32 per candidate on four voxels.  Membership e_i >= e_rc
+// and the TX x positions q_x,i are the tile's, common to every row, so
+// X_0..X_TX-1 are evaluated once per candidate when it is staged and a lane
+// spends one v_exp_f32 per candidate on TX voxels.  Membership e_i >= e_rc
 // is u_i = X_i 2^t - 1 >= 0; a block in which some |u_i| lies within the
 // guard band is walked again and the pairs of the band are re-tested in FP64,
 // correcting the sums where the FP64 test disagrees.  The parts' sums are
 // added in a fixed order through LDS at the end, after which lane (row, s)
-// holds voxels s NV .. s NV + NV - 1 of its row (NV = 4 / NQ).
+// holds voxels s NV .. s NV + NV - 1 of its row (NV = TX / NQ).
 //
 // Thin cells.  Link cells are at least 3 sigma wide along x and y (Ex <=
 // Dx/deltaM + 2 with deltaM >= 3 sigma / min(a_x, a_y)), so along those axes
@@ -104,7 +104,7 @@ struct FastParams {
 // see it.  ROW = 2 applies that rule: a candidate whose cell is not within one
 // cell of the row's along z gets weight 0 for the row.  The diagnostic kernel
 // tests the z offset of every pair when F.thin_z is set.
-template <bool DENSITY, bool DIAG, int ROW, int TZ>
+template <bool DENSITY, bool DIAG, int ROW, int TZ, int TX = 4>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
                                                    const int* __restrict__ cell_ids,
                                                    const double* __restrict__ pt_frac,
@@ -117,10 +117,12 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
 {
     static_assert(!(DIAG && ROW), "the diagnostic kernel tests every pair itself");
     static_assert(TZ == 4 || (ROW && TZ == 8), "tiles of 4 x 4 x 4, or 4 x 4 x 8 for the row kernel");
+    static_assert(TX == 4 || (ROW && TX == 8 && TZ == 4), "rows of 8 voxels: 8 x 4 x 4 tiles of the row kernel");
     // ROW: candidate parts per row (4 for 16 rows, 2 for 32 rows), voxels a
     // lane finishes (4 / NQ)
     constexpr int NQ = ROW ? (TZ == 8 ? 2 : 4) : 1;
-    constexpr int NV = ROW ? 4 / NQ : 1;
+    constexpr int NV = ROW ? TX / NQ : 1;
+    constexpr int XR = TX / 4;                          // float4 records of a candidate's X factors
     // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
     // order 0 gives each XCD a contiguous range of tiles so neighbouring
     // tiles, which read the same link cells, share one L2; order 1 is linear.
@@ -131,8 +133,8 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const VoxParams& P = F.v;
     const int tx = L % F.ntx, ty = (L / F.ntx) % F.nty, tz = L / (F.ntx * F.nty);
     const int lane = threadIdx.x;
-    const int x0 = tx * TILE, y0 = ty * TILE, z0 = tz * TZ;
-    const int x1 = min(P.Mx, x0 + TILE) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TZ) - 1;
+    const int x0 = tx * TX, y0 = ty * TILE, z0 = tz * TZ;
+    const int x1 = min(P.Mx, x0 + TX) - 1, y1 = min(P.My, y0 + TILE) - 1, z1 = min(P.Mz, z0 + TZ) - 1;
     // the lane's row of voxels along x (y fastest) and, lane per voxel, its x
     const int rrow = ROW ? lane / NQ : (lane >> 2);
     const int ix = x0 + (lane & 3), iy = y0 + (rrow & 3), iz = z0 + (rrow >> 2);
@@ -140,17 +142,18 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const bool act = ix <= x1 && row_act;
     const int jx = min(ix, x1), jy = min(iy, y1), jz = min(iz, z1);
     const QL* const qlz_tab = TZ == 8 ? P.ax.ql[3] : P.ax.ql[2];   // z relative to the tile's first cell
+    const QL* const qlx_tab = TX == 8 ? P.ax.ql[4] : P.ax.ql[0];   // x relative to the tile's first cell
 
     // origin cell of the tile; the voxel's FP32 position relative to it and
     // its cell offset (host tables)
     const int c0x = P.ax.cell[0][x0], c0y = P.ax.cell[1][y0], c0z = P.ax.cell[2][z0];
-    const QL qlx = P.ax.ql[0][jx], qly = P.ax.ql[1][jy], qlz = qlz_tab[jz];
+    const QL qlx = qlx_tab[jx], qly = P.ax.ql[1][jy], qlz = qlz_tab[jz];
     const int lcx = qlx.lc, lcy = qly.lc, lcz = qlz.lc;
     const float qx = qlx.q, qy = qly.q, qz = qlz.q;
     // the tile's voxel box: positions do not decrease with the voxel index
     // (link cells are monotonic along each axis), so its corners are the
     // first and last voxels
-    const QL ex0 = P.ax.ql[0][x0], ex1 = P.ax.ql[0][x1];
+    const QL ex0 = qlx_tab[x0], ex1 = qlx_tab[x1];
     const QL ey0 = P.ax.ql[1][y0], ey1 = P.ax.ql[1][y1];
     const QL ez0 = qlz_tab[z0], ez1 = qlz_tab[z1];
     const float bx0 = ex0.q, bx1 = ex1.q, by0 = ey0.q, by1 = ey1.q, bz0 = ez0.q, bz1 = ez1.q;
@@ -162,9 +165,9 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     const float kq2 = k2 * fmaf(qcx, qcx, fmaf(qcy, qcy, qcz * qcz));
     // ROW: the tile's four x positions (clamped at the grid's edge), the
     // lane's candidate part and the y-z part of its exponent
-    float qxt[4];
+    float qxt[TX];
 #pragma unroll
-    for (int i = 0; i < 4; i++) qxt[i] = P.ax.ql[0][min(x0 + i, x1)].q - ccx;
+    for (int i = 0; i < TX; i++) qxt[i] = qlx_tab[min(x0 + i, x1)].q - ccx;
     const int squart = lane & (NQ - 1);
     const float kyz = fmaf(k2, fmaf(qcy, qcy, qcz * qcz), -F.e_rc);
 
@@ -177,7 +180,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     // (cos, sin) of g.r, slot, cell offsets; ROW = true, (X_0..X_3), then
     // (-2 k2 p_y, -2 k2 p_z, k2 (p_y^2 + p_z^2), cos), then (sin, slot, cell
     // offsets, 0), and four zero records after the last candidate of a pass
-    constexpr int REC = ROW ? 3 : 2;
+    constexpr int REC = ROW ? XR + 2 : 2;
     // ROW: 96 staged candidates per pass keep the wave's LDS at 7.6 KB, so
     // its 88 VGPRs, not LDS, bound occupancy (20 waves per CU; 128 allowed 17
     // and ran 15 % slower on c1)
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     __shared__ float4 sA[REC * NREC];
     int* const cstart = reinterpret_cast<int*>(sA);    // kept cells' first candidate (before staging)
     static_assert(sizeof(int) * TILE_MAXC <= sizeof(float4) * REC * NREC, "cstart overlay");
-    static_assert(!ROW || 3 * 256 * sizeof(float) <= sizeof(float4) * REC * NREC, "row sums overlay");
+    static_assert(!ROW || 3 * 64 * TX * sizeof(float) <= sizeof(float4) * REC * NREC, "row sums overlay");
 
     // set bits of a wave mask below this lane (v_mbcnt)
     auto below = [](unsigned long long m) -> int {
@@ -275,9 +278,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     unsigned nguard = 0;
     // ROW: sums (with their Kahan compensations) and largest u of the row's
     // four voxels over this lane's part of the candidates
-    float rre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, rim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float kre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, kim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float umax[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+    float rre[TX], rim[TX], kre[TX], kim[TX], umax[TX];
+#pragma unroll
+    for (int i = 0; i < TX; i++) {
+        rre[i] = 0.0f; rim[i] = 0.0f; kre[i] = 0.0f; kim[i] = 0.0f; umax[i] = -1.0f;
+    }
 
     int t_next = 0, kbase = 0;
     while (t_next < T) {
@@ -295,7 +300,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             kbase += (int)__popcll(sm);
             const int t = t_next + lane;
             bool keep = false;
-            float4 r0, r1, r2;
+            float4 r0, r1, r2, r0b;
             if (t < T) {
                 const float4 cr = crec[k];
                 const int slot = t + __float_as_int(cr.w);
@@ -308,13 +313,14 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 if (keep) {
                     const float m2k = -2.0f * k2;
                     if (ROW) {
-                        float xv[4];
+                        float xv[TX];
 #pragma unroll
-                        for (int i = 0; i < 4; i++) {
+                        for (int i = 0; i < TX; i++) {
                             const float d = px - qxt[i];
                             xv[i] = __builtin_amdgcn_exp2f(k2 * (d * d));
                         }
                         r0 = make_float4(xv[0], xv[1], xv[2], xv[3]);
+                        if (TX == 8) r0b = make_float4(xv[4 % TX], xv[5 % TX], xv[6 % TX], xv[7 % TX]);
                         r1 = make_float4(m2k * py, m2k * pz, k2 * fmaf(py, py, pz * pz), pc.w);
                         r2 = make_float4(sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]), 0.0f);
                     } else {
@@ -327,8 +333,9 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             if (keep) {
                 const int p = n + below(m);
                 sA[REC * p] = r0;
-                sA[REC * p + 1] = r1;
-                if (ROW) sA[REC * p + 2] = r2;
+                if (ROW && TX == 8) sA[REC * p + 1] = r0b;
+                sA[REC * p + (ROW ? XR : 1)] = r1;
+                if (ROW) sA[REC * p + XR + 1] = r2;
             }
             n += (int)__popcll(m);
             t_next += 64;
@@ -338,9 +345,8 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             // zero records after the last candidate: X = 0, never a member
             // and never in the guard band (u = -1)
             const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            sA[REC * (n + lane)] = z4;
-            sA[REC * (n + lane) + 1] = z4;
-            sA[REC * (n + lane) + 2] = z4;
+#pragma unroll
+            for (int k = 0; k < REC; k++) sA[REC * (n + lane) + k] = z4;
         }
         __syncthreads();
         if (DIAG) {
@@ -421,27 +427,37 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             // which sets the register peak).
             const int nst = (n + NQ - 1) / NQ;
             auto step = [](float u) { return __builtin_amdgcn_fmed3f(u * 1099511627776.0f, 0.0f, 1.0f); };
+            // the X factors of staged candidate j
+            auto load_x = [&](int j, float* xv) {
+#pragma unroll
+                for (int c = 0; c < XR; c++) {
+                    const float4 X = sA[REC * j + c];
+                    xv[4 * c] = X.x; xv[4 * c + 1] = X.y; xv[4 * c + 2] = X.z; xv[4 * c + 3] = X.w;
+                }
+            };
             for (int kb = 0; kb < nst; kb += ROW_BLOCK) {
                 const int ke = min(kb + ROW_BLOCK, nst);
                 float dmin = 3.4028234663852886e38f;
-                float bre[4] = {0.0f, 0.0f, 0.0f, 0.0f}, bim[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                float bre[TX], bim[TX];
+#pragma unroll
+                for (int i = 0; i < TX; i++) { bre[i] = 0.0f; bim[i] = 0.0f; }
 #pragma unroll 2
                 for (int kk = kb; kk < ke; kk++) {
                     const int j = NQ * kk + squart;
-                    const float4 X = sA[REC * j];
-                    const float4 B = sA[REC * j + 1];
+                    float xv[TX];
+                    load_x(j, xv);
+                    const float4 B = sA[REC * j + XR];
                     float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
                     float sn;
                     if (ROW == 2) {
-                        const float4 C = sA[REC * j + 2];
+                        const float4 C = sA[REC * j + XR + 1];
                         sn = C.x;
                         E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
                     } else {
-                        sn = sA[REC * j + 2].x;
+                        sn = sA[REC * j + XR + 1].x;
                     }
-                    const float xv[4] = {X.x, X.y, X.z, X.w};
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
+                    for (int i = 0; i < TX; i++) {
                         const float u = fmaf(xv[i], E, -1.0f);
                         const float s = step(u);
                         const float w = fmaf(s, u, s);
@@ -452,26 +468,26 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     }
                 }
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
+                for (int i = 0; i < TX; i++) {
                     kahan_add(rre[i], kre[i], bre[i]);
                     kahan_add(rim[i], kim[i], bim[i]);
                 }
                 if (__ballot(dmin <= F.e_band)) {
                     for (int kk = kb; kk < ke; kk++) {
                         const int j = NQ * kk + squart;
-                        const float4 X = sA[REC * j];
-                        const float4 B = sA[REC * j + 1];
-                        const float4 C = sA[REC * j + 2];
+                        float xv[TX];
+                        load_x(j, xv);
+                        const float4 B = sA[REC * j + XR];
+                        const float4 C = sA[REC * j + XR + 1];
                         float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
                         if (ROW == 2) E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
-                        const float xv[4] = {X.x, X.y, X.z, X.w};
 #pragma unroll
-                        for (int i = 0; i < 4; i++) {
+                        for (int i = 0; i < TX; i++) {
                             const float u = fmaf(xv[i], E, -1.0f);
                             if (fabsf(u) <= F.e_band) {
                                 const int vx = min(x0 + i, x1);
                                 if (row_act && x0 + i <= x1) nguard++;
-                                const bool in = exact_at(P.ax.ql[0][vx].lc, vx, __float_as_int(C.z),
+                                const bool in = exact_at(qlx_tab[vx].lc, vx, __float_as_int(C.z),
                                                          __float_as_int(C.y));
                                 // replace the pass's s (u + 1) by the exact test's weight
                                 const float s = step(u);
@@ -491,12 +507,12 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     // the parts' sums added in part order through LDS; otherwise its own
     float vre[NV], vim[NV], vem[NV];
     if (ROW) {
-        float2* red = reinterpret_cast<float2*>(sA);                  // [64 lanes][4 voxels]
-        float* redm = reinterpret_cast<float*>(sA) + 2 * 256;         // [64 lanes][4 voxels]
+        float2* red = reinterpret_cast<float2*>(sA);                  // [64 lanes][TX voxels]
+        float* redm = reinterpret_cast<float*>(sA) + 2 * 64 * TX;     // [64 lanes][TX voxels]
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            red[4 * lane + i] = make_float2(rre[i], rim[i]);
-            if (DENSITY) redm[4 * lane + i] = umax[i];
+        for (int i = 0; i < TX; i++) {
+            red[TX * lane + i] = make_float2(rre[i], rim[i]);
+            if (DENSITY) redm[TX * lane + i] = umax[i];
         }
         __syncthreads();
         const int rb = lane & ~(NQ - 1);
@@ -506,10 +522,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             float a = 0.0f, b = 0.0f, um = -1.0f;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const float2 v = red[4 * (rb + q) + i];
+                const float2 v = red[TX * (rb + q) + i];
                 a += v.x;
                 b += v.y;
-                if (DENSITY) um = fmaxf(um, redm[4 * (rb + q) + i]);
+                if (DENSITY) um = fmaxf(um, redm[TX * (rb + q) + i]);
             }
             vre[t] = a;
             vim[t] = b;

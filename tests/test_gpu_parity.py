@@ -95,11 +95,12 @@ def test_field_fp64_membership_bitexact(case):
     assert np.max(np.abs(rho - orho)) < 1e-14
 
 
-@pytest.fixture(params=["0", "1", "2"])
+@pytest.fixture(params=["0", "1", "2", "3"])
 def fast_variant(request, monkeypatch):
     """Every FAST voxel kernel: a lane per voxel (0), a lane per row of four
     voxels and quarter of the candidates of a 4 x 4 x 4 tile (1), a lane per
-    row and half of the candidates of a 4 x 4 x 8 tile (2)."""
+    row and half of the candidates of a 4 x 4 x 8 tile (2), a lane per row of
+    eight voxels and quarter of the candidates of an 8 x 4 x 4 tile (3)."""
     monkeypatch.setenv("D2BI_FAST_VARIANT", request.param)
     return request.param
 
