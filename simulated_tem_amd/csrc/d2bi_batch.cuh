@@ -115,13 +115,13 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, Workspace& w
         hipLaunchKernelGGL(k_bin_scatter, dim3(c.nblk), dim3(BIN_BLOCK), 0, s, c.d_x, c.BP,
                            B.mask.as<uint32_t>() + c.o_mask, B.blkoff.as<int>() + c.o_blk, c.nblk,
                            w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.pt_frac.as<double>(),
-                           w.pt_gdotr.as<double>(), w.pt_p32.as<float4>(), w.pt_sn32.as<float>(),
+                           w.pt_gdotr.as<double>(), w.pt_j.as<int>(),
                            w.cell_cnt.as<int>());
     CHK(scan_exclusive_tmp(w.cell_cnt.as<int>(), w.cell_start.as<int>(), c.ncell, scan_tmp, s));
     if (c.nr > 0)
         hipLaunchKernelGGL(k_bin_place, dim3((unsigned)((c.nr + 255) / 256)), dim3(256), 0, s, c.nr,
                            w.pt_cell.as<int>(), w.pt_slot.as<int>(), w.cell_start.as<int>(), w.cell_ids.as<int>());
-    launch_bin_finish(c.ncell, c.nr, c.BP, mpc, s, w);
+    launch_bin_finish(c.ncell, c.nr, c.BP, mpc, s, w, c.d_x, B.blkoff.as<int>() + c.o_blk, c.nblk);
     HIPCHK(hipGetLastError());
     VoxParams V;
     d2bi_record dummy;
@@ -143,7 +143,7 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, Workspace& w
     static const int ocand[2] = {0, 1};
     F.order = tune_choose(tile_order_key(p, c.tab, c.dens, c.variant), "D2BI_TILE_ORDER", ocand, 2, 0, false, s,
                           [](int) {});
-    launch_tile<false>(F, variant, c.dens, w.cell_start.as<int>(), w.cell_ids.as<int>(), w.pt_frac.as<double>(),
+    launch_tile<false>(F, variant, c.dens, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
                        gp, s);
     HIPCHK(hipGetLastError());
@@ -163,14 +163,16 @@ static void branch_buffers(Workspace& w, const BranchSizes& z, F f)
 {
     f(w.pt_cell, sizeof(int) * z.max_nr);
     f(w.pt_slot, sizeof(int) * z.max_nr);
-    f(w.pt_frac, sizeof(double) * 3 * z.max_nr);
-    f(w.pt_gdotr, sizeof(double) * z.max_nr);
-    f(w.pt_p32, sizeof(float4) * z.max_nr);
-    f(w.pt_sn32, sizeof(float) * z.max_nr);
+    if (z.mode == D2BI_MODE_FP64) {
+        f(w.pt_frac, sizeof(double) * 3 * z.max_nr);
+        f(w.pt_gdotr, sizeof(double) * z.max_nr);
+        f(w.slot_id, sizeof(int) * z.max_nr);
+        f(w.slot_cs64, sizeof(double2) * z.max_nr);
+    } else {
+        f(w.pt_j, sizeof(int) * z.max_nr);
+    }
     f(w.cell_ids, sizeof(int) * z.max_nr);
-    f(w.slot_id, sizeof(int) * z.max_nr);
     f(w.slot_frac, sizeof(double4) * z.max_nr);
-    f(w.slot_cs64, sizeof(double2) * z.max_nr);
     f(w.slot_p32, sizeof(float4) * z.max_nr);
     f(w.slot_sn32, sizeof(float) * z.max_nr);
     f(w.cell_cnt, sizeof(int) * z.max_cell);

@@ -1,6 +1,6 @@
 // d2bi_bin.cuh -- link-cell binning of the replica atoms: findGdotR and
 // LinkCell3D add as a stable counting sort (classify, scan, scatter, place,
-// sort, gather).
+// sort, then gather or, FAST imaging, emit).
 //
 // One part of the single translation unit d2bi_gpu.hip, which includes the
 // parts in order so that kernels and the device helpers they call compile
@@ -152,8 +152,10 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_classify(const double* __rest
     if (threadIdx.x < P.nuvw) blk_cnt[(int64_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Pass 2: point number ii in (replica, atom) order, link cell, offset in
-// cell and g.r; per-cell arrival slot from an atomic counter.
+// Pass 2: point number ii in (replica, atom) order, link cell and a
+// per-cell arrival slot from an atomic counter; FP64 imaging and link-cell
+// queries also the offset in cell and g.r, FAST imaging the atom number
+// (its record is emitted in slot order after the sort).
 __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restrict__ x, BinParams P,
                                                            const uint32_t* __restrict__ mask,
                                                            const int* __restrict__ blk_off, int nblk,
@@ -161,8 +163,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                                                            int* __restrict__ pt_slot,
                                                            double* __restrict__ pt_frac,
                                                            double* __restrict__ pt_gdotr,
-                                                           float4* __restrict__ pt_p32,
-                                                           float* __restrict__ pt_sn32,
+                                                           int* __restrict__ pt_j,
                                                            int* __restrict__ cell_cnt)
 {
     __shared__ int run[MAX_REPL];
@@ -202,21 +203,13 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                 const int cz = P.generic ? cell_of(fl[2], P.Ez) : (int)fl[2];
                 int cell = cx + P.Ex * (cy + P.Ey * cz);
                 pt_cell[ii] = cell;
-                const double f0 = cyy[0] - fl[0], f1 = cyy[1] - fl[1], f2 = cyy[2] - fl[2];
-                pt_frac[3 * (int64_t)ii + 0] = f0;
-                pt_frac[3 * (int64_t)ii + 1] = f1;
-                pt_frac[3 * (int64_t)ii + 2] = f2;
-                const double gd = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
                 if (P.fast) {
-                    // the FAST record: offset from the cell origin in A and
-                    // (cos, sin) of g.r, evaluated in FP64 and rounded once
-                    double sn, cs;
-                    sincos(gd, &sn, &cs);
-                    pt_p32[ii] = make_float4((float)(P.cell_a[0] * f0), (float)(P.cell_a[1] * f1),
-                                             (float)(P.cell_a[2] * f2), (float)cs);
-                    pt_sn32[ii] = (float)sn;
+                    pt_j[ii] = (int)j;
                 } else {
-                    pt_gdotr[ii] = gd;
+                    pt_frac[3 * (int64_t)ii + 0] = cyy[0] - fl[0];
+                    pt_frac[3 * (int64_t)ii + 1] = cyy[1] - fl[1];
+                    pt_frac[3 * (int64_t)ii + 2] = cyy[2] - fl[2];
+                    pt_gdotr[ii] = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
                 }
                 // arrival slot: lanes of this wave landing in one cell share
                 // one atomic (the lowest such lane adds the group's size);
@@ -381,18 +374,42 @@ __global__ __launch_bounds__(256) void k_bin_gather(int64_t nr, const int* __res
     slot_sn32[s] = (float)sn;
 }
 
-// Pass 4b, FAST imaging: the point-ordered FAST records copied into slot
-// order (the FP64 offsets stay in point order for the guard re-test).
-__global__ __launch_bounds__(256) void k_bin_gather_fast(int64_t nr, const int* __restrict__ cell_ids,
-                                                         const float4* __restrict__ pt_p32,
-                                                         const float* __restrict__ pt_sn32,
-                                                         float4* __restrict__ slot_p32, float* __restrict__ slot_sn32)
+// Pass 4b, FAST imaging: each slot's point evaluated again from its atom
+// and replica (the transform of k_bin_scatter, so the same cell offsets to
+// the bit) and written in slot order: the FAST record (offset from the cell
+// origin in A and cos g.r; sin g.r beside it), rounded once from FP64, and
+// the FP64 offset in cell for the guard re-test.  The replica of point ii
+// is the last one whose first point number is <= ii (numbers run in
+// (replica, atom) order; rep_first is padded to 32 entries with INT_MAX).
+__global__ __launch_bounds__(256) void k_bin_emit_fast(int64_t nr, const int* __restrict__ cell_ids,
+                                                       const int* __restrict__ pt_j, const double* __restrict__ x,
+                                                       BinParams P, const int* __restrict__ blk_off, int nblk,
+                                                       float4* __restrict__ slot_p32, float* __restrict__ slot_sn32,
+                                                       double4* __restrict__ slot_frac)
 {
+    __shared__ int rep_first[MAX_REPL];
+    if (threadIdx.x < MAX_REPL)
+        rep_first[threadIdx.x] = threadIdx.x < P.nuvw ? blk_off[(int64_t)threadIdx.x * nblk] : 0x7fffffff;
+    __syncthreads();
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nr) return;
     const int ii = cell_ids[s];
-    slot_p32[s] = pt_p32[ii];
-    slot_sn32[s] = pt_sn32[ii];
+    const int64_t j = pt_j[ii];
+    int nn = 0;                                       // rep_first[0] = 0 <= ii
+#pragma unroll
+    for (int step = MAX_REPL / 2; step > 0; step >>= 1)
+        if (rep_first[nn + step] <= ii) nn += step;
+    const double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+    double yy[3], cyy[3];
+    bin_point(P, x3, nn, yy, cyy);
+    const double f0 = cyy[0] - floor(cyy[0]), f1 = cyy[1] - floor(cyy[1]), f2 = cyy[2] - floor(cyy[2]);
+    const double gd = (P.gg[0] * yy[0] + P.gg[1] * yy[1]) + P.gg[2] * yy[2];
+    double sn, cs;
+    sincos(gd, &sn, &cs);
+    slot_p32[s] = make_float4((float)(P.cell_a[0] * f0), (float)(P.cell_a[1] * f1), (float)(P.cell_a[2] * f2),
+                              (float)cs);
+    slot_sn32[s] = (float)sn;
+    slot_frac[s] = make_double4(f0, f1, f2, 0.0);
 }
 
 struct Binned {
@@ -401,17 +418,20 @@ struct Binned {
     int max_per_cell = 0;
 };
 
-// Pass 4 (sort within cells, then gather by slot), on workspace w.
-static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s, Workspace& w)
+// Pass 4 (sort within cells, then gather or emit by slot), on workspace w;
+// d_x, blk_off and nblk are the condition's atoms and per-(replica, block)
+// point offsets (FAST imaging).
+static void launch_bin_finish(int ncell, int64_t nr, const BinParams& P, int* mpc, hipStream_t s, Workspace& w,
+                              const double* d_x, const int* blk_off, int nblk)
 {
     const int nb = (ncell + 255) / 256;         // blk_max holds >= nb ints (sized with cell_start)
     hipLaunchKernelGGL(k_bin_sort, dim3(nb), dim3(256), 0, s, ncell, w.cell_start.as<int>(), w.cell_ids.as<int>(),
                        w.blk_max.as<int>());
     hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(256), 0, s, w.blk_max.as<int>(), nb, mpc);
     if (nr > 0 && P.fast)
-        hipLaunchKernelGGL(k_bin_gather_fast, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
-                           w.cell_ids.as<int>(), w.pt_p32.as<float4>(), w.pt_sn32.as<float>(),
-                           w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
+        hipLaunchKernelGGL(k_bin_emit_fast, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
+                           w.cell_ids.as<int>(), w.pt_j.as<int>(), d_x, P, blk_off, nblk,
+                           w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.slot_frac.as<double4>());
     else if (nr > 0)
         hipLaunchKernelGGL(k_bin_gather, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
                            w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(), P.cell_a[0],
@@ -427,6 +447,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     if (ncell64 > (int64_t)INT32_MAX / 2) return seterr(D2BI_ERR_CAPACITY, "too many link cells");
     const int ncell = (int)ncell64;
     const int nblk = (int)std::max<int64_t>(1, (n + BIN_TILE - 1) / BIN_TILE);
+    if (P.fast && n > (int64_t)INT32_MAX) return seterr(D2BI_ERR_CAPACITY, "more than 2^31 - 1 atoms");
     P.n = n;
     CHK(w.mask.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1)));
     CHK(w.blk_cnt.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
@@ -454,24 +475,23 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     const int64_t nrc = std::max<int64_t>(nr, 1);
     CHK(w.pt_cell.ensure(sizeof(int) * nrc));
     CHK(w.pt_slot.ensure(sizeof(int) * nrc));
-    CHK(w.pt_frac.ensure(sizeof(double) * 3 * nrc));
     CHK(w.cell_ids.ensure(sizeof(int) * nrc));
     CHK(w.slot_p32.ensure(sizeof(float4) * nrc));
     CHK(w.slot_sn32.ensure(sizeof(float) * nrc));
+    CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
     if (P.fast) {
-        CHK(w.pt_p32.ensure(sizeof(float4) * nrc));
-        CHK(w.pt_sn32.ensure(sizeof(float) * nrc));
+        CHK(w.pt_j.ensure(sizeof(int) * nrc));
     } else {
+        CHK(w.pt_frac.ensure(sizeof(double) * 3 * nrc));
         CHK(w.pt_gdotr.ensure(sizeof(double) * nrc));
         CHK(w.slot_id.ensure(sizeof(int) * nrc));
-        CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
         CHK(w.slot_cs64.ensure(sizeof(double2) * nrc));
     }
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(BIN_BLOCK), 0, s, d_x, P,
                            w.mask.as<uint32_t>(), w.blk_off.as<int>(), nblk, w.pt_cell.as<int>(),
                            w.pt_slot.as<int>(), w.pt_frac.as<double>(), w.pt_gdotr.as<double>(),
-                           w.pt_p32.as<float4>(), w.pt_sn32.as<float>(), w.cell_cnt.as<int>());
+                           w.pt_j.as<int>(), w.cell_cnt.as<int>());
         HIPCHK(hipGetLastError());
     }
     CHK(scan_exclusive(w.cell_cnt.as<int>(), w.cell_start.as<int>(), ncell, s, w));
@@ -481,7 +501,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
                            w.cell_ids.as<int>());
         HIPCHK(hipGetLastError());
     }
-    launch_bin_finish(ncell, nr, P, w.counters.as<int>(), s, w);
+    launch_bin_finish(ncell, nr, P, w.counters.as<int>(), s, w, d_x, w.blk_off.as<int>(), nblk);
     HIPCHK(hipGetLastError());
     out->nr = nr;
     out->ncell = ncell;

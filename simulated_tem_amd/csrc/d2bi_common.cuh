@@ -102,7 +102,7 @@ struct Workspace {
     Buf x;                      // atoms (temporary upload)
     Buf mask, blk_cnt, blk_off;
     Buf pt_cell, pt_slot, pt_frac, pt_gdotr;
-    Buf pt_p32, pt_sn32;        // FAST records in point order (FAST imaging)
+    Buf pt_j;                   // atom number of each point (FAST imaging)
     Buf cell_cnt, cell_start, cell_ids;
     Buf slot_id, slot_frac, slot_cs64, slot_p32, slot_sn32;
     Buf scan_tmp;
@@ -122,7 +122,7 @@ struct Workspace {
     size_t pinned_cap = 0;
     template <class F> void each(F f)
     {
-        Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_p32, &pt_sn32,
+        Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_j,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
                       &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &aa, &tune_img, &blur_in, &blur_tmp,

@@ -141,12 +141,12 @@ static void pick_tile(bool diag, bool dens, int variant, bool thin_z, Go go)
 }
 
 template <bool DIAG>
-static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start, const int* cell_ids,
-                        const double* pt_frac, const float4* p32, const float* sn32, float4* field, int* cnt,
+static void launch_tile(const FastParams& F, int variant, bool dens, const int* cell_start,
+                        const double4* slot_frac, const float4* p32, const float* sn32, float4* field, int* cnt,
                         float* r2m, unsigned long long* gc, hipStream_t s)
 {
     pick_tile(DIAG, dens, variant, F.thin_z != 0, [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, cell_ids, pt_frac, p32, sn32,
+        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, cell_start, slot_frac, p32, sn32,
                            field, cnt, r2m, gc);
     });
 }
@@ -268,7 +268,7 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         Fv.order = order;
         pick_tile(want_diag, dens, v, Fv.thin_z != 0, [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(8 * Fv.per_xcd), dim3(64), 0, s, Fv, w.cell_start.as<int>(),
-                               w.cell_ids.as<int>(), w.pt_frac.as<double>(), w.slot_p32.as<float4>(),
+                               w.slot_frac.as<double4>(), w.slot_p32.as<float4>(),
                                w.slot_sn32.as<float>(), w.f32_field.as<float4>(), dcnt, (float*)dr2, gp);
         });
         return D2BI_OK;

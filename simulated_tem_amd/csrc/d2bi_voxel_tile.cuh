@@ -106,8 +106,7 @@ struct FastParams {
 // tests the z offset of every pair when F.thin_z is set.
 template <bool DENSITY, bool DIAG, int ROW, int TZ, int TX = 4>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
-                                                   const int* __restrict__ cell_ids,
-                                                   const double* __restrict__ pt_frac,
+                                                   const double4* __restrict__ slot_frac,
                                                    const float4* __restrict__ p32,
                                                    const float* __restrict__ sn32,
                                                    float4* __restrict__ field,
@@ -259,11 +258,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     auto exact_at = [&](int lx, int vx, int u, int slot) -> bool {
         const int kx = (u & 255) - 1 - lx, ky = ((u >> 8) & 255) - 1 - lcy, kz = (u >> 16) - 1 - lcz;
         if (kx < -1 || kx > 1 || ky < -1 || ky > 1 || kz < -1 || kz > 1) return false;
-        const double* f = pt_frac + 3 * (int64_t)cell_ids[slot];     // point-ordered FP64 offsets
+        const double4 f = slot_frac[slot];                              // FP64 offset in cell
         const double zix = P.ax.zi[0][vx], ziy = P.ax.zi[1][jy], ziz = P.ax.zi[2][jz];
-        const double d1 = P.a[0] * (f[0] + (kx - zix));
-        const double d2 = P.a[1] * (f[1] + (ky - ziy));
-        const double d3 = P.a[2] * (f[2] + (kz - ziz));
+        const double d1 = P.a[0] * (f.x + (kx - zix));
+        const double d2 = P.a[1] * (f.y + (ky - ziy));
+        const double d3 = P.a[2] * (f.z + (kz - ziz));
         return ((d1 * d1 + d2 * d2) + d3 * d3) <= P.rc2;
     };
     auto kahan_add = [](float& s, float& c, float v) {
