@@ -140,9 +140,8 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, Workspace& w
     int variant = c.variant;
     CHK(variant_params(p, c.tab, V, &variant, &F));
     // the tile order timed eagerly before capture (nothing is timed here)
-    static const int ocand[2] = {0, 1};
-    F.order = tune_choose(tile_order_key(p, c.tab, c.dens, c.variant), "D2BI_TILE_ORDER", ocand, 2, 0, false, s,
-                          [](int) {});
+    set_tile_order(&F, tune_choose(tile_order_key(p, c.tab, c.dens, c.variant), "D2BI_TILE_ORDER", TILE_ORDERS, 3,
+                                   2, false, s, [](int) {}));
     launch_tile<false>(F, variant, c.dens, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
                        w.slot_p32.as<float4>(), w.slot_sn32.as<float>(), w.f32_field.as<float4>(), nullptr, nullptr,
                        gp, s);
@@ -437,7 +436,8 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
             return D2BI_OK;
         };
         // branch count: forced, cached, or tuned by timing one run of each
-        // count's graph (images are the same on any count)
+        // count's graph (images are the same on any count); the count that
+        // fits is kept unless another is 3 % faster
         int chosen = nbr;
         if (nbr > 1 && !std::getenv("D2BI_BATCH_BRANCHES") && tuning_on()) {
             int64_t sum_vox = 0, max_vox_c = 0, sum_n = 0, sum_p = 0;
@@ -453,13 +453,15 @@ static int run_batch(const d2bi_record* recs, const d2bi_atoms* atoms, int nRec,
                 chosen = c;
             } else {
                 float tbest = 3.4e38f;
-                for (int nb = 1; nb <= nbr; nb++) {
+                for (int q = 0; q < nbr; q++) {
+                    const int nb = q == 0 ? nbr : q;            // the default first
                     hipGraphExec_t ex = nullptr;
                     CHK(capture(nb, &ex));
                     (void)hipGraphUpload(ex, s);
                     const float t = time_launch([&] { (void)hipGraphLaunch(ex, s); }, s, 1);
-                    if (t >= 0.0f && t < tbest) {
-                        tbest = t;
+                    if (q == 0) tbest = t >= 0.0f ? 0.97f * t : 3.4e38f;
+                    if (t >= 0.0f && (q == 0 || t < tbest)) {
+                        if (q > 0) tbest = t;
                         chosen = nb;
                         if (B.exec) (void)hipGraphExecDestroy(B.exec);
                         B.exec = ex;

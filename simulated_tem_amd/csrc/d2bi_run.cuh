@@ -42,6 +42,21 @@ static int variant_tz(int variant) { return variant == 2 ? 8 : 4; }
 // Voxels along x of a variant's tiles: 8 for variant 3 (rows of 8 x 4 x 4 tiles).
 static int variant_tx(int variant) { return variant == 3 ? 8 : 4; }
 
+// The order of the FAST voxel kernel's tiles over its workgroups and the
+// grid it needs (8 per_xcd workgroups): orders 0 and 1 launch one workgroup
+// per tile (rounded up to a multiple of 8); order 2 deals whole rows of ntx
+// tiles, so each XCD gets ceil(rows / 8) rows.
+static void set_tile_order(FastParams* F, int order)
+{
+    F->order = order;
+    if (order == 2) {
+        const int rows = (F->ntiles + F->ntx - 1) / F->ntx;
+        F->per_xcd = ((rows + 7) / 8) * F->ntx;
+    } else {
+        F->per_xcd = (F->ntiles + 7) / 8;
+    }
+}
+
 // Link cells in the largest unwrapped cell range of a tx x 4 x tz tile.
 static int tile_cells(const d2bi_plan* p, const AxisTables& tab, int tz, int tx = TILE)
 {
@@ -68,8 +83,7 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     F.nty = (p->My + TILE - 1) / TILE;
     F.ntz = (p->Mz + tz - 1) / tz;
     F.ntiles = F.ntx * F.nty * F.ntz;
-    F.per_xcd = (F.ntiles + 7) / 8;
-    F.order = 0;
+    set_tile_order(&F, 2);
     // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
     // rounded once from FP64, three FMAs); +-1e-5 relative covers it 5x
     const double k2 = -p->i2s2 * 1.4426950408889634;
@@ -214,6 +228,12 @@ static int fast_variant(const d2bi_plan* p, bool dens, bool diag, bool thin_z, T
     return best;
 }
 
+// Tile orders the tuner may choose; 2 (rows dealt to the XCDs) is the
+// default, the fastest of the three on c1, c2 and c3 (up to 12 % over per-XCD
+// ranges, whose z slabs leave the XCDs given vacuum above and below a padded
+// foil idle early).
+static const int TILE_ORDERS[3] = {0, 1, 2};
+
 // Tuner key of the FAST voxel kernel's tile order: the shape and the variant
 // that runs (after the fallback of variant_params).
 static TuneKey tile_order_key(const d2bi_plan* p, const AxisTables& tab, bool dens, int variant)
@@ -262,10 +282,10 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     const bool dens = r->use_density != 0;
     // launch parameters follow the variant's tile depth; order: the tiles'
     // order over the workgroups (0 per XCD, 1 linear)
-    auto run_variant = [&](int v, int order = 0) -> int {
+    auto run_variant = [&](int v, int order = 2) -> int {
         FastParams Fv;
         CHK(variant_params(p, tab, V, &v, &Fv));
-        Fv.order = order;
+        set_tile_order(&Fv, order);
         pick_tile(want_diag, dens, v, Fv.thin_z != 0, [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(8 * Fv.per_xcd), dim3(64), 0, s, Fv, w.cell_start.as<int>(),
                                w.slot_frac.as<double4>(), w.slot_p32.as<float4>(),
@@ -287,10 +307,9 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
     if (variant < 0) return variant;
     // the tile order: timed the first time a shape is seen (each launch
     // overwrites the whole field, so the timing runs leave no trace)
-    int order = 0;
+    int order = 2;
     if (!want_diag) {
-        static const int ocand[2] = {0, 1};
-        order = tune_choose(tile_order_key(p, tab, dens, variant), "D2BI_TILE_ORDER", ocand, 2, 0, true, s,
+        order = tune_choose(tile_order_key(p, tab, dens, variant), "D2BI_TILE_ORDER", TILE_ORDERS, 3, 2, true, s,
                             [&](int o) { (void)run_variant(variant, o); });
     }
     fill_i32((int*)gp, (int64_t)(gp_bytes / sizeof(int)), 0, s);   // tuning launches counted too

@@ -10,14 +10,17 @@
 //   TUNE_INT_BLOCK   threads per workgroup of the FAST integrators (64, 128
 //                    or 256): a lane integrates one pixel column on its own
 //   TUNE_TILE_ORDER  order of the FAST voxel kernel's tiles over the
-//                    workgroups (0: contiguous ranges per XCD, 1: linear): a
-//                    wave sums one tile on its own
+//                    workgroups (0: contiguous ranges per XCD, 1: linear, 2:
+//                    rows dealt to the XCDs in turn): a wave sums one tile on
+//                    its own
 //   TUNE_BRANCHES    concurrent branches of a batch's graph (1..4): a
 //                    condition's kernels and inputs do not depend on its branch
-// Results are kept per shape for the process and, with D2BI_TUNE_CACHE=<file>,
+// A candidate replaces the default only when it runs 3 % faster, so near
+// ties keep the default.  Results are kept per shape for the process and,
+// with D2BI_TUNE_CACHE=<file>,
 // appended to that file and read back by later processes.  D2BI_TUNE=0 turns
-// tuning off (the defaults: 256 threads, per-XCD order, four branches where
-// they fit); D2BI_INT_BLOCK, D2BI_TILE_ORDER and D2BI_BATCH_BRANCHES force a
+// tuning off (the defaults: 256 threads, rows dealt to the XCDs, four
+// branches where they fit); D2BI_INT_BLOCK, D2BI_TILE_ORDER and D2BI_BATCH_BRANCHES force a
 // choice.  Nothing is timed while a graph is being captured, or on a
 // workspace that must not allocate: those launches use cached or default
 // choices.
@@ -114,24 +117,31 @@ static int tune_choose(const TuneKey& k, const char* force_env, const int* cand,
         hi = std::max(hi, cand[i]);
     }
     const int f = forced_int(force_env, lo, hi);
-    if (f >= 0) return f;
+    for (int i = 0; i < ncand; i++)
+        if (f == cand[i]) return f;                 // a forced value that is a candidate
     if (!tuning_on()) return dflt;
     const int c = tune_get(k);
     if (c >= 0) return c;
     if (!may_time || g_capturing) return dflt;
-    int best = dflt;
-    float tbest = 3.4e38f;
-    for (int i = 0; i < ncand; i++) {
-        const float t = time_launch([&] { run(cand[i]); }, s);
-        if (t < 0.0f) {
+    // another candidate replaces the default only when it is 3 % faster, so
+    // that near ties (within the run-to-run spread) keep the default
+    float t[8];
+    float tdef = 3.4e38f;
+    for (int i = 0; i < ncand && i < 8; i++) {
+        t[i] = time_launch([&] { run(cand[i]); }, s);
+        if (t[i] < 0.0f) {
             (void)hipGetLastError();
             return dflt;
         }
-        if (t < tbest) {
-            tbest = t;
+        if (cand[i] == dflt) tdef = t[i];
+    }
+    int best = dflt;
+    float tbest = 0.97f * tdef;
+    for (int i = 0; i < ncand && i < 8; i++)
+        if (cand[i] != dflt && t[i] < tbest) {
+            tbest = t[i];
             best = cand[i];
         }
-    }
     tune_put(k, best);
     return best;
 }

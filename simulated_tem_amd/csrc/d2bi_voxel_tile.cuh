@@ -34,7 +34,7 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 struct FastParams {
     VoxParams v;
     int ntx, nty, ntz;          // tiles per axis
-    int ntiles, per_xcd;
+    int ntiles, per_xcd;        // tiles; workgroups per XCD (the grid is 8 per_xcd)
     float e_rc, e_band;         // k2 (3 sigma)^2 and the guard half-width around it
     float rc_cull2;             // (3 sigma)^2 widened for the tile-box cull
     float k2;                   // -log2(e)/(2 sigma^2)
@@ -44,7 +44,7 @@ struct FastParams {
     float cellf[3];             // link-cell sides in FP32
     int thin_z;                 // link cells thinner than 3 sigma along z
     int tz;                     // tile depth in voxels (4, or 8 for the 4 x 4 x 8 row kernel)
-    int order;                  // tiles over workgroups: 0 contiguous ranges per XCD, 1 linear
+    int order;                  // tiles over workgroups: 0 contiguous ranges per XCD, 1 linear, 2 rows dealt to XCDs
 };
 
 // One wave per voxel tile: 4x4x4 voxels, or 4x4x8 for the row kernel with
@@ -124,10 +124,19 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     constexpr int XR = TX / 4;                          // float4 records of a candidate's X factors
     // XCD-aware order: consecutive workgroups go round-robin to the 8 XCDs;
     // order 0 gives each XCD a contiguous range of tiles so neighbouring
-    // tiles, which read the same link cells, share one L2; order 1 is linear.
-    // A tile's sums do not depend on the order.
+    // tiles, which read the same link cells, share one L2 (the ranges are
+    // slabs along z); order 1 is linear; order 2 deals rows of tiles along x
+    // to the XCDs in turn (row r to XCD r mod 8), so that every XCD holds
+    // rows at every depth and a row's tiles share one L2.  A tile's sums do
+    // not depend on the order.
     const int bid = blockIdx.x;
-    const int L = F.order ? bid : (bid & 7) * F.per_xcd + (bid >> 3);
+    int L;
+    if (F.order == 0) L = (bid & 7) * F.per_xcd + (bid >> 3);
+    else if (F.order == 1) L = bid;
+    else {
+        const int idx = bid >> 3, row = (idx / F.ntx) * 8 + (bid & 7);
+        L = row * F.ntx + (idx - (idx / F.ntx) * F.ntx);
+    }
     if (L >= F.ntiles) return;
     const VoxParams& P = F.v;
     const int tx = L % F.ntx, ty = (L / F.ntx) % F.nty, tz = L / (F.ntx * F.nty);

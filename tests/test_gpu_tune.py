@@ -3,7 +3,8 @@
 The tuner times launch choices that no thread's arithmetic depends on: the
 FAST integrators' workgroup size (64, 128, 256; a lane integrates one pixel
 column alone), the FAST voxel kernel's tile order over the workgroups (per-XCD
-ranges or linear; a wave sums one tile alone) and a batch graph's concurrent
+ranges, linear, or rows dealt to the XCDs in turn; a wave sums one tile alone)
+and a batch graph's concurrent
 branch count (1..4; a condition's kernels do not depend on its branch).  Every
 forced choice, the tuned choice, the untuned defaults and a choice read back
 from a tuning cache file must give the same images bit for bit, eager and
@@ -64,7 +65,7 @@ def test_forced_launch_choices_bit_identical(clean_env):
         assert np.array_equal(a, b)
     assert all(np.all(np.isfinite(a)) and np.any(a != 0) for a in ref_e)
     for block in ("64", "128", "256"):
-        for order in ("0", "1"):
+        for order in ("0", "1", "2"):
             mp.setenv("D2BI_INT_BLOCK", block)
             mp.setenv("D2BI_TILE_ORDER", order)
             print("\n[forced] block %s order %s" % (block, order), flush=True)
@@ -133,7 +134,7 @@ def test_tuning_cache_file_and_fresh_processes(tmp_path):
         if ln[0] == 1:
             assert ln[9] in (64, 128, 256)
         elif ln[0] == 2:
-            assert ln[9] in (0, 1)
+            assert ln[9] in (0, 1, 2)
         else:
             assert 1 <= ln[9] <= 3               # three conditions
     print("[process] from the cache", flush=True)
