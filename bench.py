@@ -61,10 +61,11 @@ POLY_FORM_FLOPS = 399.0              # k_integrate_poly: step matrix, per double
 POLY_POINT_FLOPS = 76.0              # k_integrate_poly: per precession point, double step and column
 STAGE_POINT_FLOPS = 140.0            # k_integrate_fast: per precession point, double step and column
 # HBM bytes per launch of k_voxel_tile from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# passes scaled by a 1 GiB copy's counts (profiles/k_voxel_tile_*_traffic.md);
-# a profiler cannot run inside the timed process, so the figure is recorded
-# per workload
-TRAFFIC_BYTES = {"c1": 2.644e9}
+# passes, scaled by a 1 GiB copy's counts, with the library's default launch
+# choices (profiles/k_voxel_tile_c1_traffic.md, profiles/c3_batch_pmc.md); a
+# profiler cannot run inside the timed process, so the figure is recorded per
+# workload
+TRAFFIC_BYTES = {"c1": 4.557e9, "c3": 4.104e9}
 SINGLE = ("c0", "c1", "c2")
 
 
@@ -235,10 +236,10 @@ def batch_main(args, imaging, synth, np):
     p0 = im0.plan(n0, pa0, npr0)
     vox0 = p0.Mx * p0.My * p0.Mz
     ach = vox0 * FLOPS_PER_VOXEL / (ms_vox * 1e-3) / 1e12
-    roof = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8>",
+    roof = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8,4>",
             "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
             "traffic": TRAFFIC_BYTES.get(args.workload),
-            "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
+            "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, default launch choices)",
             "ms_per_launch": ms_vox, "work": "%d voxels x %g flop" % (vox0, FLOPS_PER_VOXEL),
             "transcendental_frac": vox0 * TRANS_PER_VOXEL / (ms_vox * 1e-3) / PEAK_TRANS_PER_S}
     fi = integrate_flops(p0, npr0)
@@ -315,7 +316,7 @@ def single_main(name, steps, warmup, imaging, synth, np, fp64_steps=3):
         log("[bench] %s fp64: %.3e imaged voxels/s, max err %.3e" % (name, out["fp64_value"], e.max()))
     t_vox = out["ms_voxel"] * 1e-3
     ach = vox * FLOPS_PER_VOXEL / t_vox / 1e12
-    out["roofline"] = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8>",
+    out["roofline"] = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8,4>",
                        "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                        "frac": ach / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES.get(name),
                        "ms_per_launch": out["ms_voxel"]}
