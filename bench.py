@@ -357,6 +357,39 @@ def blur_only(n, reps=5, t=3.16):
                       "kernel_bytes_per_pass": 16 * n * n}), flush=True)
 
 
+def aperture_only(n_app, name="c1", angle=2e-3):
+    """The aperture sum of workload name's condition over n_app points
+    (computeD2BI_img with apertureAngle, nAperturePoints; host atoms in, host
+    image out), timed with the shared voxel pass and with a binning and voxel
+    pass per point (D2BI_APERTURE_SHARED=0), FAST and FP64; the two images of
+    a mode must agree to the bit."""
+    import numpy as np
+    from simulated_tem_amd import imaging, synth
+    imaging.init(0)
+    cfg = dict(synth.CONFIGS[name])
+    cfg["n_tomo"] = 1
+    x, box = synth.make_atoms(cfg)
+    x, conds = synth.conditions(cfg, x, box)
+    im, pa, npr = conds[0]
+    out = {"aperture_points": n_app, "workload": name, "atoms": len(x), "angle": angle}
+    for mode, reps in (("fast", 3), ("fp64", 2)):
+        imgs = {}
+        for shared in ("1", "0"):
+            os.environ["D2BI_APERTURE_SHARED"] = shared
+            im.computeD2BI_img(x, pa, npr, mode=mode, apertureAngle=angle, nAperturePoints=n_app)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                imgs[shared] = im.computeD2BI_img(x, pa, npr, mode=mode, apertureAngle=angle,
+                                                  nAperturePoints=n_app)
+            out["%s_%s_ms" % (mode, "shared" if shared == "1" else "per_point")] = \
+                1e3 * (time.perf_counter() - t0) / reps
+        os.environ.pop("D2BI_APERTURE_SHARED", None)
+        out["%s_identical" % mode] = bool(np.array_equal(imgs["1"], imgs["0"]))
+        log("[bench] aperture %s: shared %.1f ms, per point %.1f ms" %
+            (mode, out[mode + "_shared_ms"], out[mode + "_per_point_ms"]))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +404,8 @@ def main():
     ap.add_argument("--single", default="c1", help="extra single-condition workload ('' to skip)")
     ap.add_argument("--blur", type=int, default=0, metavar="N",
                     help="only blur a seeded N x N image a few times (for a kernel trace of the blur)")
+    ap.add_argument("--aperture", type=int, default=0, metavar="N",
+                    help="only time c1's aperture sum over N points, shared pass vs a pass per point")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -384,6 +419,9 @@ def main():
         return
     if args.blur:
         blur_only(args.blur)
+        return
+    if args.aperture:
+        aperture_only(args.aperture)
         return
 
     import numpy as np

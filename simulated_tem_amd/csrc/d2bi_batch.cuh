@@ -130,9 +130,10 @@ static int enqueue_rest(const Cond& c, int idx, Batch& B, int mode, Workspace& w
     double* d_img = B.img.as<double>() + c.o_img;
     if (mode == D2BI_MODE_FP64) {
         dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
-        hipLaunchKernelGGL(k_voxel_fp64, grid, dim3(256), 0, s, V, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
-                           w.slot_cs64.as<double2>(), w.f64_re.as<double>(), w.f64_im.as<double>(),
-                           w.f64_rho.as<double>(), (int*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_voxel_fp64<1>, grid, dim3(256), 0, s, V, w.cell_start.as<int>(),
+                           w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), (int64_t)0, w.f64_re.as<double>(),
+                           w.f64_im.as<double>(), (int64_t)0, w.f64_rho.as<double>(), (int*)nullptr,
+                           (double*)nullptr);
         HIPCHK(hipGetLastError());
         return launch_integrate_fp64(p, B.aa.as<double>() + c.o_aa, d_img, s, 0, 0.0, &w);
     }

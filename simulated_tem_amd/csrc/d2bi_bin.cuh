@@ -28,6 +28,7 @@ struct BinParams {
     int64_t n;                  // atoms
     int generic;                // 1: LinkCell3D add1 (no transform, no filter)
     int fast;                   // FAST imaging: FP32 records instead of g.r and the FP64 slot arrays
+    int keep_j;                 // FP64 imaging: also keep each point's atom number (aperture phases)
     double cell_a[3];           // diagonal of the link-cell matrix (FAST-mode records)
     // replica pre-classification: c ~= M x + off[nn] approximates the
     // link-cell coordinates of replica nn (products folded, so a few
@@ -203,9 +204,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void k_bin_scatter(const double* __restr
                 const int cz = P.generic ? cell_of(fl[2], P.Ez) : (int)fl[2];
                 int cell = cx + P.Ex * (cy + P.Ey * cz);
                 pt_cell[ii] = cell;
-                if (P.fast) {
-                    pt_j[ii] = (int)j;
-                } else {
+                if (P.fast | P.keep_j) pt_j[ii] = (int)j;
+                if (!P.fast) {
                     pt_frac[3 * (int64_t)ii + 0] = cyy[0] - fl[0];
                     pt_frac[3 * (int64_t)ii + 1] = cyy[1] - fl[1];
                     pt_frac[3 * (int64_t)ii + 2] = cyy[2] - fl[2];
@@ -412,6 +412,59 @@ __global__ __launch_bounds__(256) void k_bin_emit_fast(int64_t nr, const int* __
     slot_frac[s] = make_double4(f0, f1, f2, 0.0);
 }
 
+// g-vectors of up to four aperture points (findGdotR_aperture,
+// Lib_DynamicalTwoBeamImaging.f90:1269-1396: the points of a condition differ
+// only in g, so they share its binned points and take their own g.r)
+struct AppG {
+    double g[4][3];
+};
+
+// cos and sin of g_a.r for every slot and the NA aperture points of G, the
+// slot's point evaluated again from its atom and replica as k_bin_emit_fast
+// does (g.r with k_bin_scatter's expression, so the phases are those a
+// binning at g_a would give).  F64: double2 (cos, sin) at out64[a stride +
+// slot].  FAST: NA = 1 rewrites the FP32 phase of the slot's record in place
+// (slot_p32.w, slot_sn32); NA > 1 writes float2 (cos, sin) at out32[a stride
+// + slot].
+template <bool F64, int NA>
+__global__ __launch_bounds__(256) void k_app_phase(int64_t nr, const int* __restrict__ cell_ids,
+                                                   const int* __restrict__ pt_j, const double* __restrict__ x,
+                                                   BinParams P, const int* __restrict__ blk_off, int nblk, AppG G,
+                                                   int64_t stride, double2* __restrict__ out64,
+                                                   float2* __restrict__ out32, float4* __restrict__ slot_p32,
+                                                   float* __restrict__ slot_sn32)
+{
+    __shared__ int rep_first[MAX_REPL];
+    if (threadIdx.x < MAX_REPL)
+        rep_first[threadIdx.x] = threadIdx.x < P.nuvw ? blk_off[(int64_t)threadIdx.x * nblk] : 0x7fffffff;
+    __syncthreads();
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nr) return;
+    const int ii = cell_ids[s];
+    const int64_t j = pt_j[ii];
+    int nn = 0;
+#pragma unroll
+    for (int step = MAX_REPL / 2; step > 0; step >>= 1)
+        if (rep_first[nn + step] <= ii) nn += step;
+    const double x3[3] = {x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+    double yy[3], cyy[3];
+    bin_point(P, x3, nn, yy, cyy);
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+        const double gd = (G.g[a][0] * yy[0] + G.g[a][1] * yy[1]) + G.g[a][2] * yy[2];
+        double sn, cs;
+        sincos(gd, &sn, &cs);
+        if (F64) {
+            out64[a * stride + s] = make_double2(cs, sn);
+        } else if (NA == 1) {
+            reinterpret_cast<float*>(slot_p32 + s)[3] = (float)cs;
+            slot_sn32[s] = (float)sn;
+        } else {
+            out32[a * stride + s] = make_float2((float)cs, (float)sn);
+        }
+    }
+}
+
 struct Binned {
     int64_t nr = 0;
     int ncell = 0;
@@ -447,7 +500,7 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     if (ncell64 > (int64_t)INT32_MAX / 2) return seterr(D2BI_ERR_CAPACITY, "too many link cells");
     const int ncell = (int)ncell64;
     const int nblk = (int)std::max<int64_t>(1, (n + BIN_TILE - 1) / BIN_TILE);
-    if (P.fast && n > (int64_t)INT32_MAX) return seterr(D2BI_ERR_CAPACITY, "more than 2^31 - 1 atoms");
+    if ((P.fast || P.keep_j) && n > (int64_t)INT32_MAX) return seterr(D2BI_ERR_CAPACITY, "more than 2^31 - 1 atoms");
     P.n = n;
     CHK(w.mask.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1)));
     CHK(w.blk_cnt.ensure(sizeof(int) * ((int64_t)P.nuvw * nblk + 1)));
@@ -479,9 +532,8 @@ static int run_binning(const double* d_x, int64_t n, BinParams& P, Binned* out, 
     CHK(w.slot_p32.ensure(sizeof(float4) * nrc));
     CHK(w.slot_sn32.ensure(sizeof(float) * nrc));
     CHK(w.slot_frac.ensure(sizeof(double4) * nrc));
-    if (P.fast) {
-        CHK(w.pt_j.ensure(sizeof(int) * nrc));
-    } else {
+    if (P.fast || P.keep_j) CHK(w.pt_j.ensure(sizeof(int) * nrc));
+    if (!P.fast) {
         CHK(w.pt_frac.ensure(sizeof(double) * 3 * nrc));
         CHK(w.pt_gdotr.ensure(sizeof(double) * nrc));
         CHK(w.slot_id.ensure(sizeof(int) * nrc));

@@ -102,9 +102,10 @@ struct Workspace {
     Buf x;                      // atoms (temporary upload)
     Buf mask, blk_cnt, blk_off;
     Buf pt_cell, pt_slot, pt_frac, pt_gdotr;
-    Buf pt_j;                   // atom number of each point (FAST imaging)
+    Buf pt_j;                   // atom number of each point (FAST imaging, FP64 aperture sums)
     Buf cell_cnt, cell_start, cell_ids;
     Buf slot_id, slot_frac, slot_cs64, slot_p32, slot_sn32;
+    Buf app_cs;                 // per-slot phases of the aperture points of one shared pass
     Buf scan_tmp;
     Buf blk_max;                // per-block largest cell count (link-cell sort)
     Buf guard_part;             // FAST guard re-test counters, one per 128-byte line
@@ -124,7 +125,7 @@ struct Workspace {
     {
         Buf* all[] = {&x, &mask, &blk_cnt, &blk_off, &pt_cell, &pt_slot, &pt_frac, &pt_gdotr, &pt_j,
                       &cell_cnt, &cell_start, &cell_ids, &slot_id, &slot_frac, &slot_cs64,
-                      &slot_p32, &slot_sn32, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
+                      &slot_p32, &slot_sn32, &app_cs, &scan_tmp, &blk_max, &guard_part, &counters, &ax_cell, &ax_zi, &ax_start, &ax_ql, &f64_re,
                       &f64_im, &f64_rho, &f32_field, &cnt, &r2m, &img, &aa, &tune_img, &blur_in, &blur_tmp,
                       &blur_out, &blur_w, &blur_wy, &blur_k};
         for (Buf* b : all) f(*b);

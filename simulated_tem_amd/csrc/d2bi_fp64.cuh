@@ -12,12 +12,18 @@ namespace d2bi {
 
 // One thread per voxel; the 27 neighbour cells in (kz,ky,kx) order and the
 // points of each cell in insertion order, exactly the neighbour list order
-// of neighbourList1b, summed as the reference sums it (:1877-1892).
+// of neighbourList1b, summed as the reference sums it (:1877-1892).  NA > 1:
+// the aperture branch's shared neighbour loop (:1905-1922), one weight per
+// member and one sum per aperture point a, whose phases are cs64[a cs_stride
+// + slot] and whose fields go to out_re/out_im[a out_stride + voxel]; each
+// point's sum takes the same terms in the same order as an NA = 1 pass over
+// that point's phases.
+template <int NA>
 __global__ __launch_bounds__(256) void k_voxel_fp64(VoxParams P, const int* __restrict__ cell_start,
                                                     const double4* __restrict__ frac,
-                                                    const double2* __restrict__ cs64,
+                                                    const double2* __restrict__ cs64, int64_t cs_stride,
                                                     double* __restrict__ out_re,
-                                                    double* __restrict__ out_im,
+                                                    double* __restrict__ out_im, int64_t out_stride,
                                                     double* __restrict__ out_rho,
                                                     int* __restrict__ out_cnt,
                                                     double* __restrict__ out_r2m)
@@ -28,7 +34,10 @@ __global__ __launch_bounds__(256) void k_voxel_fp64(VoxParams P, const int* __re
     if (ix >= P.Mx || iy >= P.My || iz >= P.Mz) return;
     const int cx = P.ax.cell[0][ix], cy = P.ax.cell[1][iy], cz = P.ax.cell[2][iz];
     const double zx = P.ax.zi[0][ix], zy = P.ax.zi[1][iy], zz = P.ax.zi[2][iz];
-    double sre = 0.0, sim = 0.0, r2min = 3.4028234663852886e38;
+    double sre[NA], sim[NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++) { sre[a] = 0.0; sim[a] = 0.0; }
+    double r2min = 3.4028234663852886e38;
     int n = 0;
     for (int kz = -1; kz <= 1; kz++) {
         const int jz = (cz + P.Ez + kz) % P.Ez;
@@ -51,19 +60,25 @@ __global__ __launch_bounds__(256) void k_voxel_fp64(VoxParams P, const int* __re
                         n++;
                         r2min = r2min < dd ? r2min : dd;
                         const double e = exp(-dd * P.i2s2);
-                        const double2 cs = cs64[s];
-                        sre = sre + e * cs.x;
-                        sim = sim + e * cs.y;
+#pragma unroll
+                        for (int a = 0; a < NA; a++) {
+                            const double2 cs = cs64[a * cs_stride + s];
+                            sre[a] = sre[a] + e * cs.x;
+                            sim[a] = sim[a] + e * cs.y;
+                        }
                     }
                 }
             }
         }
     }
     const int64_t o = (int64_t)ix + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
-    double ww = hypot(sre, sim);
-    ww = 1 / (1.0e-16 > ww ? 1.0e-16 : ww);
-    out_re[o] = sre * ww;
-    out_im[o] = sim * ww;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+        double ww = hypot(sre[a], sim[a]);
+        ww = 1 / (1.0e-16 > ww ? 1.0e-16 : ww);
+        out_re[a * out_stride + o] = sre[a] * ww;
+        out_im[a * out_stride + o] = sim[a] * ww;
+    }
     out_rho[o] = P.use_density ? rho_of(r2min, P.sigma) : 1.0;
     if (out_cnt) out_cnt[o] = n;
     if (out_r2m) out_r2m[o] = r2min;

@@ -84,6 +84,9 @@ static int fast_params(const d2bi_plan* p, const AxisTables& tab, const VoxParam
     F.ntz = (p->Mz + tz - 1) / tz;
     F.ntiles = F.ntx * F.nty * F.ntz;
     set_tile_order(&F, 2);
+    F.app_cs = nullptr;
+    F.app_stride = 0;
+    F.field_stride = 0;
     // guard band: the FP32 exponent carries <= ~2e-6 relative error (inputs
     // rounded once from FP64, three FMAs); +-1e-5 relative covers it 5x
     const double k2 = -p->i2s2 * 1.4426950408889634;
@@ -264,9 +267,9 @@ static int run_voxel(const d2bi_record* r, const d2bi_plan* p, const AxisTables&
         CHK(w.f64_rho.ensure(sizeof(double) * nvox));
         dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
         if (T) CHK(T->record(2));
-        hipLaunchKernelGGL(k_voxel_fp64, grid, dim3(256), 0, s, V, w.cell_start.as<int>(),
-                           w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), w.f64_re.as<double>(),
-                           w.f64_im.as<double>(), w.f64_rho.as<double>(), dcnt, (double*)dr2);
+        hipLaunchKernelGGL(k_voxel_fp64<1>, grid, dim3(256), 0, s, V, w.cell_start.as<int>(),
+                           w.slot_frac.as<double4>(), w.slot_cs64.as<double2>(), (int64_t)0, w.f64_re.as<double>(),
+                           w.f64_im.as<double>(), (int64_t)0, w.f64_rho.as<double>(), dcnt, (double*)dr2);
         HIPCHK(hipGetLastError());
         if (T) CHK(T->record(3));
         return D2BI_OK;
@@ -353,7 +356,7 @@ static TuneKey int_block_key(const d2bi_plan* p, int np, bool dens)
 static bool condition_tuned(const d2bi_plan* p, const AxisTables& tab, bool dens, int variant)
 {
     if (!tuning_on()) return true;
-    if (forced_int("D2BI_TILE_ORDER", 0, 1) < 0 && tune_get(tile_order_key(p, tab, dens, variant)) < 0) return false;
+    if (forced_int("D2BI_TILE_ORDER", 0, 2) < 0 && tune_get(tile_order_key(p, tab, dens, variant)) < 0) return false;
     if (forced_int("D2BI_INT_BLOCK", 64, 256) >= 0) return true;
     for (int p0 = 0; p0 < p->nPrec;) {
         const int np = pass_points(p->nPrec - p0);
@@ -438,14 +441,16 @@ static int launch_integrate_fast(const d2bi_plan* p, bool dens, const void* f, d
 // otherwise write sum / nPrec (div = 0) or sum / div.
 // The FP64 field is read from workspace w (the library's by default).
 static int launch_integrate_fp64(const d2bi_plan* p, const double* d_aa, double* d_img, hipStream_t s,
-                                 int accumulate = 0, double div = 0.0, Workspace* wp = nullptr)
+                                 int accumulate = 0, double div = 0.0, Workspace* wp = nullptr,
+                                 int64_t field_off = 0)
 {
     Workspace& w = wp ? *wp : g.ws;
     const int64_t plane = (int64_t)p->Mx * p->My;
     IntParams P{p->Mx, p->My, p->Mz, p->nPrec, p->b0, p->bg, p->dz, d_aa, accumulate,
                 div == 0.0 ? (double)p->nPrec : div};
     hipLaunchKernelGGL(k_integrate_fp64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, P,
-                       w.f64_re.as<double>(), w.f64_im.as<double>(), w.f64_rho.as<double>(), d_img);
+                       w.f64_re.as<double>() + field_off, w.f64_im.as<double>() + field_off,
+                       w.f64_rho.as<double>(), d_img);
     HIPCHK(hipGetLastError());
     return D2BI_OK;
 }
@@ -526,15 +531,93 @@ __global__ void k_div_f64(double* __restrict__ p, int64_t n, double d)
     if (i < n) p[i] = p[i] / d;
 }
 
+// Aperture points per shared voxel pass: D2BI_APERTURE_SHARED=0 bins and
+// voxelises every point on its own instead (the same image to the bit).
+static bool aperture_shared()
+{
+    const char* e = std::getenv("D2BI_APERTURE_SHARED");
+    return !(e && e[0] == '0' && e[1] == 0);
+}
+
+// One shared voxel pass over na = 2 or 4 aperture points whose phases are in
+// w.app_cs (a stride of nr slots): FP64 fields to f64_re/f64_im at a stride of
+// nvox, FAST to f32_field at a stride of nvox records; variant is the FAST
+// row kernel (1 or 2).
+static int run_voxel_apps(const d2bi_record* r, const d2bi_plan* p, const AxisTables& tab, const AxisDev& ax,
+                          int mode, int na, int variant, int64_t nr, hipStream_t s, Workspace& w)
+{
+    const int64_t nvox = (int64_t)p->Mx * p->My * p->Mz;
+    VoxParams V;
+    fill_vox_params(r, p, ax, &V);
+    if (mode == D2BI_MODE_FP64) {
+        CHK(w.f64_re.ensure(sizeof(double) * nvox * na));
+        CHK(w.f64_im.ensure(sizeof(double) * nvox * na));
+        CHK(w.f64_rho.ensure(sizeof(double) * nvox));
+        dim3 grid((p->Mx + 7) / 8, (p->My + 7) / 8, (p->Mz + 3) / 4);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, V, w.cell_start.as<int>(), w.slot_frac.as<double4>(),
+                               w.app_cs.as<double2>(), nr, w.f64_re.as<double>(), w.f64_im.as<double>(), nvox,
+                               w.f64_rho.as<double>(), (int*)nullptr, (double*)nullptr);
+        };
+        if (na == 4) go(k_voxel_fp64<4>);
+        else go(k_voxel_fp64<2>);
+        HIPCHK(hipGetLastError());
+        return D2BI_OK;
+    }
+    const bool dens = r->use_density != 0;
+    CHK(w.f32_field.ensure((dens ? sizeof(float4) : sizeof(float2)) * nvox * na));
+    FastParams F;
+    CHK(variant_params(p, tab, V, &variant, &F));
+    if (variant != 1 && variant != 2) return seterr(D2BI_ERR_ARG, "shared aperture pass: variant %d", variant);
+    F.app_cs = w.app_cs.as<float2>();
+    F.app_stride = nr;
+    F.field_stride = nvox;
+    const int fo = forced_int("D2BI_TILE_ORDER", 0, 2);
+    set_tile_order(&F, fo >= 0 ? fo : 2);
+    CHK(w.counters.ensure(64));
+    unsigned long long* gc = (unsigned long long*)(w.counters.as<char>() + 8);
+    const size_t gp_bytes = sizeof(unsigned long long) * GUARD_SLOTS * GUARD_STRIDE;
+    CHK(w.guard_part.ensure(gp_bytes));
+    unsigned long long* gp = w.guard_part.as<unsigned long long>();
+    fill_i32((int*)gp, (int64_t)(gp_bytes / sizeof(int)), 0, s);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(8 * F.per_xcd), dim3(64), 0, s, F, w.cell_start.as<int>(),
+                           w.slot_frac.as<double4>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>(),
+                           w.f32_field.as<float4>(), (int*)nullptr, (float*)nullptr, gp);
+    };
+    auto pick = [&](auto dens_c, auto row_c, auto na_c) {
+        constexpr bool D = decltype(dens_c)::value;
+        constexpr int R = decltype(row_c)::value, N = decltype(na_c)::value;
+        if (variant == 2) go(k_voxel_tile<D, false, R, 8, 4, N>);
+        else go(k_voxel_tile<D, false, R, 4, 4, N>);
+    };
+    auto pick_na = [&](auto dens_c, auto row_c) {
+        if (na == 4) pick(dens_c, row_c, std::integral_constant<int, 4>{});
+        else pick(dens_c, row_c, std::integral_constant<int, 2>{});
+    };
+    auto pick_row_c = [&](auto dens_c) {
+        if (F.thin_z) pick_na(dens_c, std::integral_constant<int, 2>{});
+        else pick_na(dens_c, std::integral_constant<int, 1>{});
+    };
+    if (dens) pick_row_c(std::true_type{});
+    else pick_row_c(std::false_type{});
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_guard, dim3(1), dim3(256), 0, s, gp, GUARD_SLOTS, GUARD_STRIDE, gc);
+    return D2BI_OK;
+}
+
 // computeD2BI_img with apertureAngle and nAperturePoints > 1
 // (Lib_DynamicalTwoBeamImaging.f90:1775-1786, 1853-1866, 1905-1951).  Every
 // aperture point has its own g-vector for g.r (findGdotR_aperture :1269-1396)
 // and so its own phase field; the link cells, ranges and densities are those
-// of the unshifted condition.  Each point is binned, voxelised and integrated
-// at its nPrec deviation parameters aa(app, :), the column sums are added up
-// and the image is divided by nApp * nPrec at the end (the reference adds
-// over (prec, app) in the other nesting; FP64 sums agree to rounding).
-// Result left in ws.img.
+// of the unshifted condition.  The atoms are binned once; each pass then
+// evaluates the phases of up to four points per slot (k_app_phase) and
+// voxelises them together, one neighbour walk for all of them as the
+// reference's loop over the neighbour list does (:1905-1922), and each
+// point's field is integrated at its nPrec deviation parameters aa(app, :).
+// The column sums are added up point by point and the image is divided by
+// nApp * nPrec at the end (the reference adds over (prec, app) in the other
+// nesting; FP64 sums agree to rounding).  Result left in ws.img.
 static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, double precAngle, int nPrec,
                           double apAngle, int nApp, int mode, d2bi_stats* st, d2bi_plan* plan_out)
 {
@@ -550,41 +633,121 @@ static int image_aperture(const d2bi_record* r, const double* d_x, int64_t n, do
     AxisDev ax;
     CHK(upload_axis(tab, &ax, s, w));
     const int64_t plane = (int64_t)plan.Mx * plan.My;
+    const int64_t nvox = plane * plan.Mz;
     CHK(w.img.ensure(sizeof(double) * plane));
     CHK(w.aa.ensure(sizeof(double) * D2BI_MAX_PREC));
     double* d_img = w.img.as<double>();
     const bool dens = r->use_density != 0;
     const double total = (double)nApp * nPrec;
-    for (int app = 0; app < nApp; app++) {
+    const bool f64 = mode == D2BI_MODE_FP64;
+    // the plan of aperture point app: its g and deviation parameters
+    auto point_plan = [&](int app) {
         d2bi_plan pk = plan;
         for (int i = 0; i < 3; i++) pk.gg[i] = gshift[3 * app + i];
         for (int q = 0; q < nPrec; q++) pk.aa[q] = aa[app + (size_t)nApp * q];
-        BinParams BP;
-        CHK(bin_params(r, &pk, &BP));
-        BP.fast = mode == D2BI_MODE_FAST ? 1 : 0;
-        Binned b;
-        CHK(run_binning(d_x, n, BP, &b, s, w));
-        CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s, w));
-        if (mode == D2BI_MODE_FP64) {
+        return pk;
+    };
+    // aperture point app's field (FP64: offset into f64_re/f64_im; FAST: the
+    // field record array) integrated into d_img
+    auto integrate_point = [&](int app, int64_t f64_off, const void* f32) -> int {
+        const d2bi_plan pk = point_plan(app);
+        if (f64) {
             double* d_aa = w.aa.as<double>();
             HIPCHK(hipMemcpyAsync(d_aa, pk.aa, sizeof(double) * nPrec, hipMemcpyHostToDevice, s));
-            // app 0 writes its sum (div 1), the others add theirs
-            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0, &w));
+            // point 0 writes its sum (div 1), the others add theirs
+            CHK(launch_integrate_fp64(&pk, d_aa, d_img, s, app > 0 ? 1 : 0, 1.0, &w, f64_off));
+            // pk.aa (host) is read by the copy above: finish before it goes
+            HIPCHK(hipStreamSynchronize(s));
         } else {
-            CHK(launch_integrate_fast(&pk, dens, w.f32_field.p, d_img, s, app > 0, 1.0 / total, &w));
+            CHK(launch_integrate_fast(&pk, dens, f32, d_img, s, app > 0, 1.0 / total, &w));
         }
+        return D2BI_OK;
+    };
+    auto add_stats = [&](const Binned& b, int points) -> int {
         int64_t hc[2] = {0, 0};
         if (st) HIPCHK(hipMemcpyAsync(hc, w.counters.p, 16, hipMemcpyDeviceToHost, s));
-        // pk.aa (host) is read by the copy above: finish before the next point
         HIPCHK(hipStreamSynchronize(s));
         if (st) {
-            st->n_points += b.nr;
-            st->n_guard += mode == D2BI_MODE_FAST ? hc[1] : 0;
+            st->n_points += b.nr * points;
+            st->n_guard += f64 ? 0 : hc[1];
             st->max_per_cell = std::max(st->max_per_cell, (int)(hc[0] & 0xffffffff));
             st->n_cells = b.ncell;
         }
+        return D2BI_OK;
+    };
+
+    if (!aperture_shared() || (f64 && n > (int64_t)INT32_MAX)) {
+        // every point binned and voxelised on its own
+        for (int app = 0; app < nApp; app++) {
+            d2bi_plan pk = point_plan(app);
+            BinParams BP;
+            CHK(bin_params(r, &pk, &BP));
+            BP.fast = f64 ? 0 : 1;
+            Binned b;
+            CHK(run_binning(d_x, n, BP, &b, s, w));
+            CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s, w));
+            CHK(integrate_point(app, 0, w.f32_field.p));
+            CHK(add_stats(b, 1));
+        }
+    } else {
+        // one binning, at point 0's g (its phases are the binning's own)
+        d2bi_plan p0 = point_plan(0);
+        BinParams BP;
+        CHK(bin_params(r, &p0, &BP));
+        BP.fast = f64 ? 0 : 1;
+        BP.keep_j = f64 ? 1 : 0;
+        Binned b;
+        CHK(run_binning(d_x, n, BP, &b, s, w));
+        const int64_t nr = b.nr;
+        // shared passes of four or two points: FP64, or the FAST row kernels
+        // of rows of four voxels (the default variant 2, or a forced 1 or 2)
+        int variant = forced_variant();
+        if (variant < 0) variant = 2;
+        variant = effective_variant(&plan, tab, variant);
+        const int na_max = (f64 || variant == 1 || variant == 2) ? 4 : 1;
+        if (na_max > 1) CHK(w.app_cs.ensure((f64 ? sizeof(double2) : sizeof(float2)) * std::max<int64_t>(nr, 1) * na_max));
+        const int nblk = (int)std::max<int64_t>(1, (n + BIN_TILE - 1) / BIN_TILE);
+        for (int app = 0; app < nApp;) {
+            const int left = nApp - app;
+            const int na = na_max >= 4 && left >= 4 ? 4 : (na_max >= 2 && left >= 2 ? 2 : 1);
+            if (nr > 0 && (app > 0 || na > 1)) {
+                AppG G;
+                std::memset(&G, 0, sizeof G);
+                for (int a = 0; a < na; a++)
+                    for (int i = 0; i < 3; i++) G.g[a][i] = gshift[3 * (app + a) + i];
+                const dim3 grid((unsigned)((nr + 255) / 256));
+                auto go = [&](auto kern) {
+                    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, nr, w.cell_ids.as<int>(), w.pt_j.as<int>(), d_x,
+                                       BP, w.blk_off.as<int>(), nblk, G, nr,
+                                       na > 1 ? w.app_cs.as<double2>() : w.slot_cs64.as<double2>(),
+                                       w.app_cs.as<float2>(), w.slot_p32.as<float4>(), w.slot_sn32.as<float>());
+                };
+                if (f64) {
+                    if (na == 4) go(k_app_phase<true, 4>);
+                    else if (na == 2) go(k_app_phase<true, 2>);
+                    else go(k_app_phase<true, 1>);
+                } else {
+                    if (na == 4) go(k_app_phase<false, 4>);
+                    else if (na == 2) go(k_app_phase<false, 2>);
+                    else go(k_app_phase<false, 1>);
+                }
+                HIPCHK(hipGetLastError());
+            }
+            if (na == 1) {
+                const d2bi_plan pk = point_plan(app);
+                CHK(run_voxel(r, &pk, tab, ax, mode, false, nullptr, s, w));
+                CHK(integrate_point(app, 0, w.f32_field.p));
+            } else {
+                CHK(run_voxel_apps(r, &plan, tab, ax, mode, na, variant, nr, s, w));
+                const size_t rec = dens ? sizeof(float4) : sizeof(float2);
+                for (int a = 0; a < na; a++)
+                    CHK(integrate_point(app + a, nvox * a, w.f32_field.as<char>() + rec * nvox * a));
+            }
+            CHK(add_stats(b, na));
+            app += na;
+        }
     }
-    if (mode == D2BI_MODE_FP64) {
+    if (f64) {
         hipLaunchKernelGGL(k_div_f64, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, d_img, plane, total);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));

@@ -45,6 +45,11 @@ struct FastParams {
     int thin_z;                 // link cells thinner than 3 sigma along z
     int tz;                     // tile depth in voxels (4, or 8 for the 4 x 4 x 8 row kernel)
     int order;                  // tiles over workgroups: 0 contiguous ranges per XCD, 1 linear, 2 rows dealt to XCDs
+    // aperture points sharing one pass (kernels with NA > 1): point a's
+    // (cos, sin) of g.r at app_cs[a app_stride + slot], its field at
+    // field[a field_stride + voxel]
+    const float2* app_cs;
+    int64_t app_stride, field_stride;
 };
 
 // One wave per voxel tile: 4x4x4 voxels, or 4x4x8 for the row kernel with
@@ -104,7 +109,16 @@ struct FastParams {
 // see it.  ROW = 2 applies that rule: a candidate whose cell is not within one
 // cell of the row's along z gets weight 0 for the row.  The diagnostic kernel
 // tests the z offset of every pair when F.thin_z is set.
-template <bool DENSITY, bool DIAG, int ROW, int TZ, int TX = 4>
+//
+// Aperture points (NA = 2 or 4, row kernels with rows of four voxels): the NA
+// points of a condition's aperture sum share its binned points, weights and
+// membership and differ only in g.r (Lib_DynamicalTwoBeamImaging.f90:
+// 1905-1922 sums them in one neighbour loop).  Each staged candidate carries
+// NA (cos, sin) pairs, every step adds the pair's weight into NA sums, and a
+// guard correction is applied to each.  Point a's sums take the same terms in
+// the same order as an NA = 1 pass with that point's phases, so its field is
+// that pass's to the bit.
+template <bool DENSITY, bool DIAG, int ROW, int TZ, int TX = 4, int NA = 1>
 __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __restrict__ cell_start,
                                                    const double4* __restrict__ slot_frac,
                                                    const float4* __restrict__ p32,
@@ -117,6 +131,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     static_assert(!(DIAG && ROW), "the diagnostic kernel tests every pair itself");
     static_assert(TZ == 4 || (ROW && TZ == 8), "tiles of 4 x 4 x 4, or 4 x 4 x 8 for the row kernel");
     static_assert(TX == 4 || (ROW && TX == 8 && TZ == 4), "rows of 8 voxels: 8 x 4 x 4 tiles of the row kernel");
+    static_assert(NA == 1 || ((NA == 2 || NA == 4) && ROW && TX == 4), "aperture points: 2 or 4, row kernels of 4");
     // ROW: candidate parts per row (4 for 16 rows, 2 for 32 rows), voxels a
     // lane finishes (4 / NQ)
     constexpr int NQ = ROW ? (TZ == 8 ? 2 : 4) : 1;
@@ -187,8 +202,10 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     // staged candidates: ROW = false, (-2 k2 p, k2 |p|^2) tile-centred, then
     // (cos, sin) of g.r, slot, cell offsets; ROW = true, (X_0..X_3), then
     // (-2 k2 p_y, -2 k2 p_z, k2 (p_y^2 + p_z^2), cos), then (sin, slot, cell
-    // offsets, 0), and four zero records after the last candidate of a pass
-    constexpr int REC = ROW ? XR + 2 : 2;
+    // offsets, 0), NA > 1 then (cos, sin) of the aperture points two to a
+    // record, and four zero records after the last candidate of a pass
+    constexpr int NPH = NA > 1 ? NA / 2 : 0;
+    constexpr int REC = ROW ? XR + 2 + NPH : 2;
     // ROW: 96 staged candidates per pass keep the wave's LDS at 7.6 KB, so
     // its 88 VGPRs, not LDS, bound occupancy (20 waves per CU; 128 allowed 17
     // and ran 15 % slower on c1)
@@ -284,12 +301,16 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
     float emax = -3.4028234663852886e38f;
     int cnt = 0;
     unsigned nguard = 0;
-    // ROW: sums (with their Kahan compensations) and largest u of the row's
-    // four voxels over this lane's part of the candidates
-    float rre[TX], rim[TX], kre[TX], kim[TX], umax[TX];
+    // ROW: sums per aperture point (with their Kahan compensations) and
+    // largest u of the row's voxels over this lane's part of the candidates
+    float rre[NA][TX], rim[NA][TX], kre[NA][TX], kim[NA][TX], umax[TX];
 #pragma unroll
     for (int i = 0; i < TX; i++) {
-        rre[i] = 0.0f; rim[i] = 0.0f; kre[i] = 0.0f; kim[i] = 0.0f; umax[i] = -1.0f;
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            rre[a][i] = 0.0f; rim[a][i] = 0.0f; kre[a][i] = 0.0f; kim[a][i] = 0.0f;
+        }
+        umax[i] = -1.0f;
     }
 
     int t_next = 0, kbase = 0;
@@ -308,7 +329,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
             kbase += (int)__popcll(sm);
             const int t = t_next + lane;
             bool keep = false;
-            float4 r0, r1, r2, r0b;
+            float4 r0, r1, r2, r0b, rph[NPH > 0 ? NPH : 1];
             if (t < T) {
                 const float4 cr = crec[k];
                 const int slot = t + __float_as_int(cr.w);
@@ -330,7 +351,14 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                         r0 = make_float4(xv[0], xv[1], xv[2], xv[3]);
                         if (TX == 8) r0b = make_float4(xv[4 % TX], xv[5 % TX], xv[6 % TX], xv[7 % TX]);
                         r1 = make_float4(m2k * py, m2k * pz, k2 * fmaf(py, py, pz * pz), pc.w);
-                        r2 = make_float4(sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]), 0.0f);
+                        r2 = make_float4(NA == 1 ? sn32[slot] : 0.0f, __int_as_float(slot),
+                                         __int_as_float(cupk[k]), 0.0f);
+#pragma unroll
+                        for (int h = 0; h < NPH; h++) {
+                            const float2 c0 = F.app_cs[(2 * h) * F.app_stride + slot];
+                            const float2 c1 = F.app_cs[(2 * h + 1) * F.app_stride + slot];
+                            rph[h] = make_float4(c0.x, c0.y, c1.x, c1.y);
+                        }
                     } else {
                         r0 = make_float4(m2k * px, m2k * py, m2k * pz, k2 * fmaf(px, px, fmaf(py, py, pz * pz)));
                         r1 = make_float4(pc.w, sn32[slot], __int_as_float(slot), __int_as_float(cupk[k]));
@@ -344,6 +372,8 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 if (ROW && TX == 8) sA[REC * p + 1] = r0b;
                 sA[REC * p + (ROW ? XR : 1)] = r1;
                 if (ROW) sA[REC * p + XR + 1] = r2;
+#pragma unroll
+                for (int h = 0; h < NPH; h++) sA[REC * p + XR + 2 + h] = rph[h];
             }
             n += (int)__popcll(m);
             t_next += 64;
@@ -443,12 +473,28 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     xv[4 * c] = X.x; xv[4 * c + 1] = X.y; xv[4 * c + 2] = X.z; xv[4 * c + 3] = X.w;
                 }
             };
+            // the (cos, sin) pairs of staged candidate j (B = its record XR,
+            // sn its sine when NA = 1)
+            auto load_ph = [&](int j, const float4& B, float sn, float* pc, float* ps) {
+                if (NA == 1) {
+                    pc[0] = B.w;
+                    ps[0] = sn;
+                } else {
+#pragma unroll
+                    for (int h = 0; h < NPH; h++) {
+                        const float4 Ph = sA[REC * j + XR + 2 + h];
+                        pc[2 * h] = Ph.x; ps[2 * h] = Ph.y; pc[2 * h + 1] = Ph.z; ps[2 * h + 1] = Ph.w;
+                    }
+                }
+            };
             for (int kb = 0; kb < nst; kb += ROW_BLOCK) {
                 const int ke = min(kb + ROW_BLOCK, nst);
                 float dmin = 3.4028234663852886e38f;
-                float bre[TX], bim[TX];
+                float bre[NA][TX], bim[NA][TX];
 #pragma unroll
-                for (int i = 0; i < TX; i++) { bre[i] = 0.0f; bim[i] = 0.0f; }
+                for (int i = 0; i < TX; i++)
+#pragma unroll
+                    for (int a = 0; a < NA; a++) { bre[a][i] = 0.0f; bim[a][i] = 0.0f; }
 #pragma unroll 2
                 for (int kk = kb; kk < ke; kk++) {
                     const int j = NQ * kk + squart;
@@ -456,30 +502,37 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     load_x(j, xv);
                     const float4 B = sA[REC * j + XR];
                     float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
-                    float sn;
+                    float sn = 0.0f;
                     if (ROW == 2) {
                         const float4 C = sA[REC * j + XR + 1];
                         sn = C.x;
                         E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
-                    } else {
+                    } else if (NA == 1) {
                         sn = sA[REC * j + XR + 1].x;
                     }
+                    float pc[NA], ps[NA];
+                    load_ph(j, B, sn, pc, ps);
 #pragma unroll
                     for (int i = 0; i < TX; i++) {
                         const float u = fmaf(xv[i], E, -1.0f);
                         const float s = step(u);
                         const float w = fmaf(s, u, s);
-                        bre[i] = fmaf(w, B.w, bre[i]);
-                        bim[i] = fmaf(w, sn, bim[i]);
+#pragma unroll
+                        for (int a = 0; a < NA; a++) {
+                            bre[a][i] = fmaf(w, pc[a], bre[a][i]);
+                            bim[a][i] = fmaf(w, ps[a], bim[a][i]);
+                        }
                         dmin = fminf(dmin, fabsf(u));
                         if (DENSITY) umax[i] = fmaxf(umax[i], u);
                     }
                 }
 #pragma unroll
-                for (int i = 0; i < TX; i++) {
-                    kahan_add(rre[i], kre[i], bre[i]);
-                    kahan_add(rim[i], kim[i], bim[i]);
-                }
+                for (int i = 0; i < TX; i++)
+#pragma unroll
+                    for (int a = 0; a < NA; a++) {
+                        kahan_add(rre[a][i], kre[a][i], bre[a][i]);
+                        kahan_add(rim[a][i], kim[a][i], bim[a][i]);
+                    }
                 if (__ballot(dmin <= F.e_band)) {
                     for (int kk = kb; kk < ke; kk++) {
                         const int j = NQ * kk + squart;
@@ -494,14 +547,20 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                             const float u = fmaf(xv[i], E, -1.0f);
                             if (fabsf(u) <= F.e_band) {
                                 const int vx = min(x0 + i, x1);
-                                if (row_act && x0 + i <= x1) nguard++;
+                                // one re-test per aperture point it settles
+                                if (row_act && x0 + i <= x1) nguard += NA;
                                 const bool in = exact_at(qlx_tab[vx].lc, vx, __float_as_int(C.z),
                                                          __float_as_int(C.y));
                                 // replace the pass's s (u + 1) by the exact test's weight
                                 const float s = step(u);
                                 const float d = (in ? u + 1.0f : 0.0f) - fmaf(s, u, s);
-                                kahan_add(rre[i], kre[i], d * B.w);
-                                kahan_add(rim[i], kim[i], d * C.x);
+                                float pc[NA], ps[NA];
+                                load_ph(j, B, C.x, pc, ps);
+#pragma unroll
+                                for (int a = 0; a < NA; a++) {
+                                    kahan_add(rre[a][i], kre[a][i], d * pc[a]);
+                                    kahan_add(rim[a][i], kim[a][i], d * ps[a]);
+                                }
                             }
                         }
                     }
@@ -513,36 +572,41 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
 
     // the lane's voxels: ROW, voxels s NV .. s NV + NV - 1 of its row, from
     // the parts' sums added in part order through LDS; otherwise its own
-    float vre[NV], vim[NV], vem[NV];
+    float vre[NA][NV], vim[NA][NV], vem[NV];
     if (ROW) {
         float2* red = reinterpret_cast<float2*>(sA);                  // [64 lanes][TX voxels]
         float* redm = reinterpret_cast<float*>(sA) + 2 * 64 * TX;     // [64 lanes][TX voxels]
-#pragma unroll
-        for (int i = 0; i < TX; i++) {
-            red[TX * lane + i] = make_float2(rre[i], rim[i]);
-            if (DENSITY) redm[TX * lane + i] = umax[i];
-        }
-        __syncthreads();
         const int rb = lane & ~(NQ - 1);
 #pragma unroll
-        for (int t = 0; t < NV; t++) {
-            const int i = squart * NV + t;
-            float a = 0.0f, b = 0.0f, um = -1.0f;
+        for (int ap = 0; ap < NA; ap++) {
+            if (ap > 0) __syncthreads();                              // the previous point's reads
 #pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const float2 v = red[TX * (rb + q) + i];
-                a += v.x;
-                b += v.y;
-                if (DENSITY) um = fmaxf(um, redm[TX * (rb + q) + i]);
+            for (int i = 0; i < TX; i++) {
+                red[TX * lane + i] = make_float2(rre[ap][i], rim[ap][i]);
+                if (DENSITY && ap == 0) redm[TX * lane + i] = umax[i];
             }
-            vre[t] = a;
-            vim[t] = b;
-            // largest exponent in the t frame: 2^t = u + 1 (u = -1: no candidate)
-            vem[t] = (DENSITY && um > -1.0f) ? __builtin_amdgcn_logf(um + 1.0f) : -3.4028234663852886e38f;
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < NV; t++) {
+                const int i = squart * NV + t;
+                float a = 0.0f, b = 0.0f, um = -1.0f;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const float2 v = red[TX * (rb + q) + i];
+                    a += v.x;
+                    b += v.y;
+                    if (DENSITY && ap == 0) um = fmaxf(um, redm[TX * (rb + q) + i]);
+                }
+                vre[ap][t] = a;
+                vim[ap][t] = b;
+                // largest exponent in the t frame: 2^t = u + 1 (u = -1: no candidate)
+                if (ap == 0)
+                    vem[t] = (DENSITY && um > -1.0f) ? __builtin_amdgcn_logf(um + 1.0f) : -3.4028234663852886e38f;
+            }
         }
     } else {
-        vre[0] = sre;
-        vim[0] = sim;
+        vre[0][0] = sre;
+        vim[0][0] = sim;
         vem[0] = emax;
     }
 
@@ -551,13 +615,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
         const int vx = ROW ? x0 + squart * NV + t : ix;
         if (!(ROW ? vx <= x1 && row_act : act)) continue;
         const int64_t o = (int64_t)vx + (int64_t)P.Mx * (iy + (int64_t)P.My * iz);
-        const float sre_ = vre[t], sim_ = vim[t], emax_ = vem[t];
-        // x = S / max(|S|, 1e-16).  Without DIAG the sums carry the factor
-        // 2^-e_rc of the t frame (weights 2^t = 2^(e - e_rc)); it cancels in
-        // S/|S| and moves the threshold.
-        const float m2 = fmaf(sre_, sre_, sim_ * sim_);
-        const float ww = m2 > (DIAG ? 1.0e-32f : F.m2_min) ? __builtin_amdgcn_rsqf(m2)
-                                                          : (DIAG ? 1.0e16f : 1.0e16f * F.w_scale);
+        const float emax_ = vem[t];
         // nearest squared range from the largest exponent (huge(1.0) if none).
         // Without DIAG the largest exponent is over every candidate, members
         // or not: it differs from the members' only when no candidate is
@@ -576,8 +634,19 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                 rh = 1.0f - xx * xx * xx * fmaf(6.0f * xx, xx, fmaf(-15.0f, xx, 10.0f));
             }
         }
-        if (DENSITY) field[o] = make_float4(sre_ * ww, sim_ * ww, rh, 0.0f);
-        else reinterpret_cast<float2*>(field)[o] = make_float2(sre_ * ww, sim_ * ww);
+#pragma unroll
+        for (int ap = 0; ap < NA; ap++) {
+            const float sre_ = vre[ap][t], sim_ = vim[ap][t];
+            // x = S / max(|S|, 1e-16).  Without DIAG the sums carry the factor
+            // 2^-e_rc of the t frame (weights 2^t = 2^(e - e_rc)); it cancels in
+            // S/|S| and moves the threshold.
+            const float m2 = fmaf(sre_, sre_, sim_ * sim_);
+            const float ww = m2 > (DIAG ? 1.0e-32f : F.m2_min) ? __builtin_amdgcn_rsqf(m2)
+                                                              : (DIAG ? 1.0e16f : 1.0e16f * F.w_scale);
+            const int64_t oa = o + ap * F.field_stride;
+            if (DENSITY) field[oa] = make_float4(sre_ * ww, sim_ * ww, rh, 0.0f);
+            else reinterpret_cast<float2*>(field)[oa] = make_float2(sre_ * ww, sim_ * ww);
+        }
         if (DIAG) {
             out_cnt[o] = cnt;
             out_r2m[o] = r2m;

@@ -192,6 +192,35 @@ def test_image_aperture(n_app, ap_angle, density, fast_variant):
         assert np.array_equal(gf, im.computeD2BI_img(x, 0.005, 3, mode="fast"))
 
 
+@pytest.mark.parametrize("n_app,density,geom", [(7, True, "k111"), (8, False, "k111"), (6, False, "thin_z")])
+def test_image_aperture_shared_pass(n_app, density, geom, fast_variant, monkeypatch):
+    """The aperture sum's shared pass (one binning, the phases of groups of
+    four and two points per slot, one neighbour walk per group) against a
+    binning and voxel pass per point (D2BI_APERTURE_SHARED=0): the same image
+    to the bit in both modes, and the same statistics.  n = 7 is a group of
+    four, one of two and a single point; thin z cells take the row kernel with
+    the z-offset test.  Variants 0 and 3 have no shared kernel and walk once
+    per point over the one binning."""
+    if geom == "k111":
+        x, im = _cond("c0", cells=(9, 11, 7), g=(1, -1, 0), k=(1, 1, 1), d=(25.0, 27.0, 19.0), sg=0.01,
+                      density=density)
+    else:
+        x, im = _cond("c0", cells=(12, 12, 12), d=(37.9824, 37.9824, 37.8), sg=0.01, density=density)
+    out = {}
+    for shared in ("1", "0"):
+        monkeypatch.setenv("D2BI_APERTURE_SHARED", shared)
+        for mode in ("fp64", "fast"):
+            st = abi.D2biStats()
+            img = im.computeD2BI_img(x, 0.005, 3, mode=mode, apertureAngle=2e-3, nAperturePoints=n_app, stats=st)
+            out[shared, mode] = (img, st.n_points, st.n_guard, st.max_per_cell, st.n_cells)
+    print("[aperture shared] n=%d %s v%s guard fast %d" % (n_app, geom, fast_variant, out["1", "fast"][2]),
+          flush=True)
+    for mode in ("fp64", "fast"):
+        a, b = out["1", mode], out["0", mode]
+        assert np.array_equal(a[0], b[0]), mode
+        assert a[1:] == b[1:], mode
+
+
 def test_blur_matches_oracle():
     rng = np.random.default_rng(7)
     # windows cut by both edges, one row or column, a height that is no
