@@ -236,7 +236,7 @@ def batch_main(args, imaging, synth, np):
     p0 = im0.plan(n0, pa0, npr0)
     vox0 = p0.Mx * p0.My * p0.Mz
     ach = vox0 * FLOPS_PER_VOXEL / (ms_vox * 1e-3) / 1e12
-    roof = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8,4>",
+    roof = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8,4,1>",
             "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
             "traffic": TRAFFIC_BYTES.get(args.workload),
             "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, default launch choices)",
@@ -316,7 +316,7 @@ def single_main(name, steps, warmup, imaging, synth, np, fp64_steps=3):
         log("[bench] %s fp64: %.3e imaged voxels/s, max err %.3e" % (name, out["fp64_value"], e.max()))
     t_vox = out["ms_voxel"] * 1e-3
     ach = vox * FLOPS_PER_VOXEL / t_vox / 1e12
-    out["roofline"] = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8,4>",
+    out["roofline"] = {"bound": "valu", "pipe": "valu_fp32", "kernel": "k_voxel_tile<false,false,1,8,4,1>",
                        "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                        "frac": ach / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES.get(name),
                        "ms_per_launch": out["ms_voxel"]}
