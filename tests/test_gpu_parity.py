@@ -85,6 +85,23 @@ def test_linkcells_bitexact(case):
     assert np.all(frac >= 0) and np.all(frac < 1)
 
 
+def test_binning_after_smaller_and_larger_conditions():
+    """The workspace's point and slot arrays grow for a larger condition and
+    are reused, larger than needed, by a smaller one.  Small, large, small,
+    large in one process: the link cells and the FAST and FP64 images of each
+    equal the oracle's (cells bit for bit) every time."""
+    small = _cond("c0")
+    large = _cond("c1", cells=(22, 18, 12))
+    for x, im in (small, large, small, large):
+        oc = orc.Condition(im.rec, x)
+        cs, ids, frac, gd = im.linkcells(x)
+        ocs, oids, ofrac = oc.cells()
+        assert np.array_equal(cs, ocs) and np.array_equal(ids, oids) and np.array_equal(frac, ofrac)
+        ref = orc.Condition(im.rec, x, 0.005, 2).image(nthreads=NTH)
+        assert np.max(np.abs(im.computeD2BI_img(x, 0.005, 2, mode="fp64") - ref)) <= FP64_TOL
+        assert np.max(np.abs(im.computeD2BI_img(x, 0.005, 2, mode="fast") - ref)) <= FAST_TOL
+
+
 def test_field_fp64_membership_bitexact(case):
     name, x, im, oc = case
     ph, rho, cnt, r2m = im.field(x, "fp64")
