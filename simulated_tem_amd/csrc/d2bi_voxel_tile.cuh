@@ -14,7 +14,7 @@ static constexpr int TILE_CAP = 160;                 // staged candidates per pa
 static constexpr int GUARD_SLOTS = 1024;             // guard counters (power of two) ...
 static constexpr int GUARD_STRIDE = 16;              // ... 16 x 8 B = 128 B apart
 static constexpr int TILE_MAXC = 128;                // cells in a tile's unwrapped range (<= 5^3)
-static constexpr int ROW_BLOCK = 16;                 // row kernel: steps per guard test and Kahan merge
+static constexpr int ROW_BLOCK = 32;                 // row kernel: steps per guard test and Kahan merge
 
 // Inclusive prefix sum over the 64 lanes of a wave in six DPP steps: within
 // each row of 16 lanes (row_shr 1, 2, 4, 8 with zero fill), then row 0's
