@@ -88,11 +88,12 @@ struct FastParams {
 // for TZ = 4, 32 for TZ = 8) times NQ candidate parts (4 or 2); lane (row, s)
 // walks candidates s, s+NQ, s+2NQ, ... for the TX voxels of its row (TX = 4,
 // or 8 for 8 x 4 x 4 tiles).  The Gaussian factorises along x:
-//     2^(e_i - e_rc) = X_i * 2^t,   X_i = 2^(k2 (p_x - q_x,i)^2),
-//     t = k2 ((p_y - q_y)^2 + (p_z - q_z)^2) - e_rc   (e_rc = k2 (3 sigma)^2),
+//     2^(e_i - e_rc) = X_i * 2^t,   X_i = 2^(k2 ((p_x - q_x,i)^2 + p_y^2 + p_z^2)),
+//     t = -2 k2 (p_y q_y + p_z q_z) + k2 (q_y^2 + q_z^2) - e_rc   (e_rc = k2 (3 sigma)^2),
 // and the TX x positions q_x,i are the tile's, common to every row, so
-// X_0..X_TX-1 are evaluated once per candidate when it is staged and a lane
-// spends one v_exp_f32 per candidate on TX voxels.  Membership e_i >= e_rc
+// X_0..X_TX-1 (which carry the candidate's own y-z term) are evaluated once
+// per candidate when it is staged and a lane spends two FMAs and one
+// v_exp_f32 per candidate on TX voxels.  Membership e_i >= e_rc
 // is u_i = X_i 2^t - 1 >= 0; a block in which some |u_i| lies within the
 // guard band is walked again and the pairs of the band are re-tested in FP64,
 // correcting the sums where the FP64 test disagrees.  The parts' sums are
@@ -201,9 +202,9 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
 
     // staged candidates: ROW = false, (-2 k2 p, k2 |p|^2) tile-centred, then
     // (cos, sin) of g.r, slot, cell offsets; ROW = true, (X_0..X_3), then
-    // (-2 k2 p_y, -2 k2 p_z, k2 (p_y^2 + p_z^2), cos), then (sin, slot, cell
-    // offsets, 0), NA > 1 then (cos, sin) of the aperture points two to a
-    // record, and four zero records after the last candidate of a pass
+    // (-2 k2 p_y, -2 k2 p_z, sin, cos), then (sin, slot, cell offsets, 0),
+    // NA > 1 then (cos, sin) of the aperture points two to a record, and four
+    // zero records after the last candidate of a pass
     constexpr int NPH = NA > 1 ? NA / 2 : 0;
     constexpr int REC = ROW ? XR + 2 + NPH : 2;
     // ROW: 96 staged candidates per pass keep the wave's LDS at 7.6 KB, so
@@ -343,14 +344,15 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     const float m2k = -2.0f * k2;
                     if (ROW) {
                         float xv[TX];
+                        const float kyzp = k2 * fmaf(py, py, pz * pz);
 #pragma unroll
                         for (int i = 0; i < TX; i++) {
                             const float d = px - qxt[i];
-                            xv[i] = __builtin_amdgcn_exp2f(k2 * (d * d));
+                            xv[i] = __builtin_amdgcn_exp2f(fmaf(k2, d * d, kyzp));
                         }
                         r0 = make_float4(xv[0], xv[1], xv[2], xv[3]);
                         if (TX == 8) r0b = make_float4(xv[4 % TX], xv[5 % TX], xv[6 % TX], xv[7 % TX]);
-                        r1 = make_float4(m2k * py, m2k * pz, k2 * fmaf(py, py, pz * pz), pc.w);
+                        r1 = make_float4(m2k * py, m2k * pz, NA == 1 ? sn32[slot] : 0.0f, pc.w);
                         r2 = make_float4(NA == 1 ? sn32[slot] : 0.0f, __int_as_float(slot),
                                          __int_as_float(cupk[k]), 0.0f);
 #pragma unroll
@@ -501,14 +503,11 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                     float xv[TX];
                     load_x(j, xv);
                     const float4 B = sA[REC * j + XR];
-                    float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
-                    float sn = 0.0f;
+                    float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, kyz)));
+                    const float sn = B.z;
                     if (ROW == 2) {
                         const float4 C = sA[REC * j + XR + 1];
-                        sn = C.x;
                         E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
-                    } else if (NA == 1) {
-                        sn = sA[REC * j + XR + 1].x;
                     }
                     float pc[NA], ps[NA];
                     load_ph(j, B, sn, pc, ps);
@@ -540,7 +539,7 @@ __global__ __launch_bounds__(64) void k_voxel_tile(FastParams F, const int* __re
                         load_x(j, xv);
                         const float4 B = sA[REC * j + XR];
                         const float4 C = sA[REC * j + XR + 1];
-                        float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, B.z + kyz)));
+                        float E = __builtin_amdgcn_exp2f(fmaf(B.x, qcy, fmaf(B.y, qcz, kyz)));
                         if (ROW == 2) E = (unsigned)((__float_as_int(C.z) >> 16) - lcz) <= 2u ? E : 0.0f;
 #pragma unroll
                         for (int i = 0; i < TX; i++) {
